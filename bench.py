@@ -1,0 +1,189 @@
+"""Benchmark: motion-estimation frames/s at 1080p with 4 references.
+
+Workload (one GPU): 1920x1080 scene video (48 moving rectangles, global pan
+(+0.75, +0.25) px/frame, Gaussian noise sigma 2), hierarchical GOP4 B coding
+order.  A step searches the 8 coded pictures at coding positions 5..12
+(P8 B6 B5 B7 P12 B10 B9 B11), each holding a full 4-slot picture buffer: every
+in-frame quadtree node (64/32/16/8) against all 4 references, centred on the
+block position, search range 32 (B) / 24 (P).  References are the source
+frames (open loop).  Pictures are resident in HBM before timing starts; a
+step is one launch per frame on one stream.
+
+Prints one JSON line (see the repo's measurement notes for field meanings).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+W, H = 1920, 1080
+DISPLAY_FRAMES = 13
+TIMED_EMITS = range(5, 13)
+
+
+def build_workload(eng):
+    from tcodec_amd import synth, workload
+    scene = synth.Scene(W, H, seed=2026, rects=48, pan=(0.75, 0.25))
+    pics = {}
+    for t in range(DISPLAY_FRAMES):
+        pics[t] = eng.pic(scene.frame(t)[0])
+    order = workload.gop4_coding_order(DISPLAY_FRAMES)
+    nodes = workload.qt_nodes(W, H)
+    frames = []
+    for e in TIMED_EMITS:
+        poc, kind, dpb = order[e]
+        assert len(dpb) == 4, (poc, dpb)
+        sr = workload.SR_B if kind == "B" else workload.SR_P
+        frames.append((poc, kind, workload.frame_calls(nodes, [pics[p] for p in dpb], pics[poc], sr)))
+    return scene, pics, frames
+
+
+def cpu_baseline(scene, pics, frames, budget_s=12.0):
+    """The reference's own tc_motion_est (compiled from its sources into
+    oracle/_ref/libtcref.so), one thread, on a seeded sample of one frame's
+    calls; returns ME frames/s extrapolated from calls/s."""
+    import oracle
+    if not oracle.reference_available():
+        return None
+    R = oracle.Reference()
+    poc, kind, calls = frames[0]
+    dpb_pocs = sorted(set(calls["ref_pic"].tolist()))
+    rng = np.random.default_rng(0)
+    idx = rng.permutation(len(calls))
+    planes = {}
+    poc_of = {pid: t for t, pid in pics.items()}
+
+    def plane(pic_id):
+        if pic_id not in planes:
+            planes[pic_id] = scene.frame(poc_of[pic_id])[0]
+        return planes[pic_id]
+
+    cur = plane(int(calls["cur_pic"][0]))
+    t0 = time.perf_counter()
+    done = 0
+    for i in idx:
+        c = calls[i]
+        R.motion_est(plane(int(c["ref_pic"])), cur, int(c["px"]), int(c["py"]), int(c["cx"]),
+                     int(c["cy"]), int(c["blk"]), int(c["sr"]))
+        done += 1
+        if done % 256 == 0 and time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    calls_per_s = done / dt
+    return {"value": calls_per_s / len(calls), "unit": "ME frames/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"{done} random calls of frame POC {poc} ({kind}, {len(calls)} calls, "
+                      f"refs {dpb_pocs}) through tc_motion_est from src/motion.c, "
+                      f"{dt:.1f} s, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by stream/frame band, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from tcodec_amd import me, workload
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+    eng = me.Engine(0)
+    scene, pics, frames = build_workload(eng)
+    torch.cuda.synchronize()
+
+    d_calls, d_res, d_stats = [], [], []
+    for poc, kind, calls in frames:
+        t = torch.from_numpy(calls.view(np.int32).reshape(-1, 8).copy()).to(dev)
+        d_calls.append(t)
+        d_res.append(torch.zeros((len(calls), 4), dtype=torch.int32, device=dev))
+        d_stats.append(torch.zeros((len(calls), 4), dtype=torch.int32, device=dev))
+
+    def step(with_stats=False, events=None):
+        for i, (poc, kind, calls) in enumerate(frames):
+            if events is not None:
+                events[i][0].record(stream)
+            eng.search_device(d_calls[i].data_ptr(), len(calls), d_res[i].data_ptr(),
+                              d_stats[i].data_ptr() if with_stats else None, sptr)
+            if events is not None:
+                events[i][1].record(stream)
+
+    # statistics pass (not timed): algorithmic bytes per frame
+    step(with_stats=True)
+    torch.cuda.synchronize()
+    algo = []
+    for i, (poc, kind, calls) in enumerate(frames):
+        st = d_stats[i].cpu().numpy().view(me.STATS_DTYPE).ravel()
+        algo.append(workload.algo_bytes(st, calls["blk"]))
+        assert (d_res[i][:, 3] == 0).all().item()
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    per_frame_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in frames]
+        step(events=ev)
+        per_frame_ms.append(ev)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lat = np.array([a.elapsed_time(b) for evs in per_frame_ms for a, b in evs])
+    nframes = len(frames) * args.steps
+    fps = nframes / elapsed
+    kernel_s = lat.mean() / 1e3
+    achieved = float(np.mean(algo)) / kernel_s / 1e9
+    calls_per_frame = int(np.mean([len(c) for _, _, c in frames]))
+
+    cpu = None if args.no_cpu else cpu_baseline(scene, pics, frames)
+    out = {
+        "metric": "ME frames/s @1080p 4-ref + p50/p99 frame latency (mixed load), bit-exact",
+        "value": round(fps, 2),
+        "unit": "ME frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "1080p B-GOP4 4-ref every-node ME (open loop, source refs)",
+                   "width": W, "height": H, "frames_per_step": len(frames),
+                   "calls_per_frame": calls_per_frame, "refs_per_frame": 4,
+                   "search_range": {"P": workload.SR_P, "B": workload.SR_B}},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3),
+                       "load": "back-to-back per-frame launches, one stream"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "algo_bytes_per_frame": int(np.mean(algo)),
+                     "kernel": "me_search_kernel", "kernel_ms_avg": round(kernel_s * 1e3, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,111 @@
+/*
+ * tcme.h - MI355X (gfx950) motion-estimation backend for TCodec.
+ *
+ * Two faces, one library (libtcme.so):
+ *
+ * 1. The drop-in.  tc_motion_est() has exactly the reference's signature and
+ *    result (include/tcodec_common.h:454-459, src/motion.c:384-485 in the
+ *    reference tree).  Link it in place of motion.c's own definition and the
+ *    encoder's .tcv output is unchanged.  It never falls back to the CPU: if
+ *    no GPU can be used it prints the reason and aborts.
+ *
+ * 2. The batch API (tcme_*).  Pictures live in device memory; a call list
+ *    (any mix of references, current pictures, block sizes and search ranges)
+ *    is searched in one launch on a caller-supplied HIP stream.  Call lists,
+ *    results and statistics are device pointers.  No function here allocates
+ *    inside tcme_search(); pictures are allocated by tcme_pic_create().
+ *
+ * All entry points are C ABI with plain pointers and sizes.  Errors are
+ * negative return codes; tcme_last_error() gives the message (thread-local).
+ */
+#ifndef TCME_H
+#define TCME_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- drop-in boundary ------------------------------------------------- */
+
+/* Same layout as the reference's tc_mv_s (include/tcodec_types.h:168-171). */
+typedef struct tcme_mv { int32_t x, y; } tcme_mv;
+
+/* Replaces tc_motion_est, reference src/motion.c:384-389 (prototype
+ * include/tcodec_common.h:454-459).  Returns the absolute quarter-pel
+ * position of the best n x n block and, if best_sad_out is not NULL, its SAD.
+ * blk_size must be 8, 16, 32 or 64 (every size the reference encoder uses). */
+tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, int ref_h,
+                      const uint8_t *orig, int orig_stride, int center_x, int center_y,
+                      int blk_size, int search_range, int32_t *best_sad_out);
+
+/* ---- batch API ---------------------------------------------------------- */
+
+typedef struct tcme_ctx tcme_ctx;
+
+/* One search: reference picture, current picture, block origin (px,py) in
+ * the current picture, full-pel centre, block size, search range.  Mirrors
+ * the arguments of the reference call with the two planes named by id. */
+typedef struct tcme_call {
+    int32_t ref_pic, cur_pic;
+    int32_t px, py;
+    int32_t cx, cy;
+    int32_t blk, sr;
+} tcme_call;
+
+typedef struct tcme_result {
+    int32_t mv_x, mv_y;   /* absolute quarter-pel position */
+    int32_t sad;
+    int32_t status;       /* 0 ok, <0 rejected descriptor (see TCME_E_*) */
+} tcme_result;
+
+/* Per-call candidate counts in the reference's sequential order, used for
+ * the algorithmic-byte figures: in-frame integer SADs, 6-tap quarter-pel
+ * SADs, bilinear quarter-pel SADs, out-of-frame integer candidates. */
+typedef struct tcme_stats {
+    int32_t int_evals, q6_evals, qbil_evals, int_oob;
+} tcme_stats;
+
+enum {
+    TCME_OK = 0,
+    TCME_E_ARG = -1,
+    TCME_E_HIP = -2,
+    TCME_E_NOMEM = -3,
+    TCME_E_PIC = -4,
+    TCME_E_BLK = -5
+};
+
+const char *tcme_last_error(void);
+const char *tcme_version(void);
+
+/* Context on a device (-1 = current).  max_pics bounds the picture table. */
+int  tcme_create(tcme_ctx **out, int device, int max_pics);
+void tcme_destroy(tcme_ctx *ctx);
+
+/* Allocates a w x h luma picture with its half-sample planes on the device. */
+int  tcme_pic_create(tcme_ctx *ctx, int w, int h);
+int  tcme_pic_destroy(tcme_ctx *ctx, int pic);
+/* Fills a picture from host (upload) or device memory, then derives its raw
+ * 6-tap half-sample planes.  Asynchronous on stream (a hipStream_t). */
+int  tcme_pic_upload(tcme_ctx *ctx, int pic, const uint8_t *host, int stride, void *stream);
+int  tcme_pic_upload_device(tcme_ctx *ctx, int pic, const uint8_t *dev, int stride,
+                            void *stream);
+/* Device address and pitch of a picture's luma plane (for callers that
+ * write pixels themselves; call tcme_pic_finish afterwards). */
+int  tcme_pic_luma(tcme_ctx *ctx, int pic, uint8_t **dev, int *pitch);
+int  tcme_pic_finish(tcme_ctx *ctx, int pic, void *stream);
+
+/* Searches n calls (device array) into results (device array); stats may be
+ * NULL.  Asynchronous on stream.  Returns TCME_OK once launched. */
+int  tcme_search(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,
+                 tcme_stats *stats, void *stream);
+
+/* Host-pointer convenience: copies calls in, searches, copies results out,
+ * synchronises the stream. */
+int  tcme_search_host(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,
+                      tcme_stats *stats, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

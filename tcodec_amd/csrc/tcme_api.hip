@@ -1,0 +1,454 @@
+// Host runtime and C ABI of libtcme.so (declared in include/tcme.h).
+//
+// Batch API: a context owns a device picture table; pictures are luma planes
+// with their half-sample planes; tcme_search() launches one wavefront per call.
+//
+// Drop-in: tc_motion_est() keeps device mirrors of the reference planes it is
+// handed.  Before each call it compares, byte for byte, the part of the host
+// plane that call can read (the reachable search window plus the six-tap
+// apron) with the mirror's host shadow and re-uploads on any difference, so a
+// result is always computed from exactly the bytes the caller passed, however
+// the caller reuses its buffers.  The current block travels with the call.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "tcme_internal.h"
+
+#define TCME_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(TCME_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),      \
+                        __FILE__, __LINE__);                                            \
+    } while (0)
+
+struct Pic {
+    bool used = false;
+    int w = 0, h = 0, pitch = 0, hpitch = 0;
+    uint8_t *d_y = nullptr;
+    int16_t *d_half = nullptr;  // hh | vv | cc, each hpitch*h
+};
+
+int align_up(int v, int a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+struct tcme_ctx {
+    int device = 0;
+    int max_pics = 0;
+    std::vector<Pic> pics;
+    tcme::PicDev *d_table = nullptr;
+    // staging for tcme_search_host (grown on demand)
+    tcme_call *d_calls = nullptr;
+    tcme_result *d_res = nullptr;
+    tcme_stats *d_stats = nullptr;
+    size_t stage_cap = 0;
+    std::mutex mu;
+};
+
+namespace {
+
+int set_device(tcme_ctx *c)
+{
+    HIP_TRY(hipSetDevice(c->device));
+    return TCME_OK;
+}
+
+tcme::PicDev dev_view(const Pic &p)
+{
+    tcme::PicDev d{};
+    if (!p.used) return d;
+    d.y = p.d_y;
+    d.hh = p.d_half;
+    d.vv = p.d_half ? p.d_half + static_cast<size_t>(p.hpitch) * p.h : nullptr;
+    d.cc = p.d_half ? p.d_half + 2 * static_cast<size_t>(p.hpitch) * p.h : nullptr;
+    d.w = p.w;
+    d.h = p.h;
+    d.pitch = p.pitch;
+    d.hpitch = p.hpitch;
+    return d;
+}
+
+int check_pic(tcme_ctx *c, int pic)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    if (pic < 0 || pic >= c->max_pics || !c->pics[pic].used)
+        return fail(TCME_E_PIC, "picture %d does not exist", pic);
+    return TCME_OK;
+}
+
+int pic_alloc(tcme_ctx *c, int w, int h, bool half)
+{
+    if (w <= 0 || h <= 0 || w > 16384 || h > 16384)
+        return fail(TCME_E_ARG, "picture size %dx%d out of range", w, h);
+    int id = -1;
+    for (int i = 0; i < c->max_pics; ++i)
+        if (!c->pics[i].used) { id = i; break; }
+    if (id < 0) return fail(TCME_E_NOMEM, "picture table full (%d)", c->max_pics);
+    Pic p;
+    p.w = w;
+    p.h = h;
+    p.pitch = align_up(w, 64);
+    p.hpitch = align_up(w, 64);
+    size_t ybytes = static_cast<size_t>(p.pitch) * h + tcme::kTailSlack;
+    if (hipMalloc(&p.d_y, ybytes) != hipSuccess)
+        return fail(TCME_E_NOMEM, "hipMalloc luma %zu bytes", ybytes);
+    hipMemset(p.d_y, 0, ybytes);
+    if (half) {
+        size_t hbytes = 3 * static_cast<size_t>(p.hpitch) * h * sizeof(int16_t) + tcme::kTailSlack;
+        if (hipMalloc(&p.d_half, hbytes) != hipSuccess) {
+            hipFree(p.d_y);
+            return fail(TCME_E_NOMEM, "hipMalloc half-sample planes %zu bytes", hbytes);
+        }
+        hipMemset(p.d_half, 0, hbytes);
+    }
+    p.used = true;
+    c->pics[id] = p;
+    tcme::PicDev d = dev_view(p);
+    HIP_TRY(hipMemcpy(c->d_table + id, &d, sizeof d, hipMemcpyHostToDevice));
+    return id;
+}
+
+int pic_free(tcme_ctx *c, int id)
+{
+    Pic &p = c->pics[id];
+    hipFree(p.d_y);
+    if (p.d_half) hipFree(p.d_half);
+    p = Pic();
+    tcme::PicDev d{};
+    HIP_TRY(hipMemcpy(c->d_table + id, &d, sizeof d, hipMemcpyHostToDevice));
+    return TCME_OK;
+}
+
+int pic_derive(tcme_ctx *c, int id, hipStream_t s)
+{
+    const Pic &p = c->pics[id];
+    if (!p.d_half) return TCME_OK;
+    HIP_TRY(tcme::launch_halfpel(dev_view(p), s));
+    return TCME_OK;
+}
+
+}  // namespace
+
+TCME_API const char *tcme_last_error(void) { return g_err.c_str(); }
+
+TCME_API const char *tcme_version(void) { return "tcme 0.1 gfx950"; }
+
+TCME_API int tcme_create(tcme_ctx **out, int device, int max_pics)
+{
+    if (!out) return fail(TCME_E_ARG, "null out");
+    *out = nullptr;
+    if (max_pics <= 0) max_pics = 1024;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(TCME_E_HIP, "no HIP device available");
+    if (device < 0) HIP_TRY(hipGetDevice(&device));
+    if (device >= ndev) return fail(TCME_E_ARG, "device %d >= count %d", device, ndev);
+    tcme_ctx *c = new tcme_ctx();
+    c->device = device;
+    c->max_pics = max_pics;
+    c->pics.resize(max_pics);
+    if (set_device(c) != TCME_OK) { delete c; return TCME_E_HIP; }
+    if (hipMalloc(&c->d_table, sizeof(tcme::PicDev) * max_pics) != hipSuccess) {
+        delete c;
+        return fail(TCME_E_NOMEM, "picture table");
+    }
+    hipMemset(c->d_table, 0, sizeof(tcme::PicDev) * max_pics);
+    *out = c;
+    return TCME_OK;
+}
+
+TCME_API void tcme_destroy(tcme_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipDeviceSynchronize();
+    for (int i = 0; i < c->max_pics; ++i)
+        if (c->pics[i].used) pic_free(c, i);
+    hipFree(c->d_table);
+    hipFree(c->d_calls);
+    hipFree(c->d_res);
+    hipFree(c->d_stats);
+    delete c;
+}
+
+TCME_API int tcme_pic_create(tcme_ctx *c, int w, int h)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    return pic_alloc(c, w, h, true);
+}
+
+TCME_API int tcme_pic_destroy(tcme_ctx *c, int pic)
+{
+    if (int e = check_pic(c, pic)) return e;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    HIP_TRY(hipDeviceSynchronize());
+    return pic_free(c, pic);
+}
+
+TCME_API int tcme_pic_upload(tcme_ctx *c, int pic, const uint8_t *host, int stride, void *stream)
+{
+    if (int e = check_pic(c, pic)) return e;
+    const Pic &p = c->pics[pic];
+    if (!host || stride < p.w) return fail(TCME_E_ARG, "bad host plane");
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipMemcpy2DAsync(p.d_y, p.pitch, host, stride, p.w, p.h, hipMemcpyHostToDevice, s));
+    return pic_derive(c, pic, s);
+}
+
+TCME_API int tcme_pic_upload_device(tcme_ctx *c, int pic, const uint8_t *dev, int stride,
+                                    void *stream)
+{
+    if (int e = check_pic(c, pic)) return e;
+    const Pic &p = c->pics[pic];
+    if (!dev || stride < p.w) return fail(TCME_E_ARG, "bad device plane");
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipMemcpy2DAsync(p.d_y, p.pitch, dev, stride, p.w, p.h, hipMemcpyDeviceToDevice, s));
+    return pic_derive(c, pic, s);
+}
+
+TCME_API int tcme_pic_luma(tcme_ctx *c, int pic, uint8_t **dev, int *pitch)
+{
+    if (int e = check_pic(c, pic)) return e;
+    if (!dev || !pitch) return fail(TCME_E_ARG, "null output");
+    *dev = c->pics[pic].d_y;
+    *pitch = c->pics[pic].pitch;
+    return TCME_OK;
+}
+
+TCME_API int tcme_pic_finish(tcme_ctx *c, int pic, void *stream)
+{
+    if (int e = check_pic(c, pic)) return e;
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    return pic_derive(c, pic, static_cast<hipStream_t>(stream));
+}
+
+TCME_API int tcme_search(tcme_ctx *c, const tcme_call *calls, int n, tcme_result *results,
+                         tcme_stats *stats, void *stream)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    if (n < 0 || (n > 0 && (!calls || !results))) return fail(TCME_E_ARG, "bad call list");
+    if (n == 0) return TCME_OK;
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    HIP_TRY(tcme::launch_search(c->d_table, c->max_pics, calls, n, results, stats,
+                                static_cast<hipStream_t>(stream)));
+    return TCME_OK;
+}
+
+TCME_API int tcme_search_host(tcme_ctx *c, const tcme_call *calls, int n, tcme_result *results,
+                              tcme_stats *stats, void *stream)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    if (n < 0 || (n > 0 && (!calls || !results))) return fail(TCME_E_ARG, "bad call list");
+    if (n == 0) return TCME_OK;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (static_cast<size_t>(n) > c->stage_cap) {
+        HIP_TRY(hipStreamSynchronize(s));
+        hipFree(c->d_calls);
+        hipFree(c->d_res);
+        hipFree(c->d_stats);
+        c->d_calls = nullptr; c->d_res = nullptr; c->d_stats = nullptr; c->stage_cap = 0;
+        size_t cap = std::max<size_t>(n, 4096);
+        if (hipMalloc(&c->d_calls, cap * sizeof(tcme_call)) != hipSuccess ||
+            hipMalloc(&c->d_res, cap * sizeof(tcme_result)) != hipSuccess ||
+            hipMalloc(&c->d_stats, cap * sizeof(tcme_stats)) != hipSuccess)
+            return fail(TCME_E_NOMEM, "staging for %d calls", n);
+        c->stage_cap = cap;
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_calls, calls, n * sizeof(tcme_call), hipMemcpyHostToDevice, s));
+    HIP_TRY(tcme::launch_search(c->d_table, c->max_pics, c->d_calls, n, c->d_res,
+                                stats ? c->d_stats : nullptr, s));
+    HIP_TRY(hipMemcpyAsync(results, c->d_res, n * sizeof(tcme_result), hipMemcpyDeviceToHost, s));
+    if (stats)
+        HIP_TRY(hipMemcpyAsync(stats, c->d_stats, n * sizeof(tcme_stats), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return TCME_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in tc_motion_est
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Mirror {
+    const uint8_t *host = nullptr;
+    int w = 0, h = 0, stride = 0;
+    int pic = -1;
+    std::vector<uint8_t> shadow;  // w*h copy of what the device holds
+    uint64_t last_use = 0;
+};
+
+struct DropIn {
+    std::mutex mu;
+    tcme_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<Mirror> mirrors;
+    uint64_t tick = 0;
+    int cur_pic = -1;                 // 64x64 block picture for the current block
+    uint8_t *h_stage = nullptr;       // pinned: call descriptor, then block rows
+    tcme_result *h_res = nullptr;     // pinned
+    tcme_call *d_call = nullptr;
+    tcme_result *d_res = nullptr;
+    uint8_t *d_blk = nullptr;         // luma of cur_pic
+    int blk_pitch = 0;
+    long calls = 0, uploads = 0;
+};
+
+DropIn g_drop;
+constexpr int kMirrors = 8;
+
+[[noreturn]] void die(const char *what)
+{
+    fprintf(stderr, "tcme: fatal: %s: %s\n", what, tcme_last_error());
+    abort();
+}
+
+void dropin_init()
+{
+    DropIn &d = g_drop;
+    const char *dev = getenv("TCME_DEVICE");
+    if (tcme_create(&d.ctx, dev ? atoi(dev) : 0, 64) != TCME_OK) die("no usable GPU");
+    if (hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess) die("stream");
+    d.cur_pic = pic_alloc(d.ctx, tcme::kMaxBlk, tcme::kMaxBlk, false);
+    if (d.cur_pic < 0) die("block picture");
+    d.d_blk = d.ctx->pics[d.cur_pic].d_y;
+    d.blk_pitch = d.ctx->pics[d.cur_pic].pitch;
+    if (hipHostMalloc(reinterpret_cast<void **>(&d.h_stage), 256 + 64 * 64, 0) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&d.h_res), sizeof(tcme_result), 0) != hipSuccess ||
+        hipMalloc(&d.d_call, sizeof(tcme_call)) != hipSuccess ||
+        hipMalloc(&d.d_res, sizeof(tcme_result)) != hipSuccess)
+        die("drop-in staging");
+}
+
+// Device picture holding the bytes of host plane `ref` that the call reading
+// rows [y0,y1] x cols [x0,x1] depends on.
+int mirror_for(const uint8_t *ref, int stride, int w, int h, int x0, int y0, int x1, int y1)
+{
+    DropIn &d = g_drop;
+    Mirror *m = nullptr;
+    for (auto &e : d.mirrors)
+        if (e.host == ref && e.w == w && e.h == h && e.stride == stride) { m = &e; break; }
+    bool stale = true;
+    if (m) {
+        stale = false;
+        const size_t span = static_cast<size_t>(x1 - x0 + 1);
+        for (int y = y0; y <= y1 && !stale; ++y)
+            stale = memcmp(m->shadow.data() + static_cast<size_t>(y) * w + x0,
+                           ref + static_cast<size_t>(y) * stride + x0, span) != 0;
+    } else {
+        if (static_cast<int>(d.mirrors.size()) < kMirrors) {
+            d.mirrors.emplace_back();
+            m = &d.mirrors.back();
+        } else {
+            m = &*std::min_element(d.mirrors.begin(), d.mirrors.end(),
+                                   [](const Mirror &a, const Mirror &b) { return a.last_use < b.last_use; });
+        }
+        if (m->pic >= 0 && (m->w != w || m->h != h)) {
+            hipStreamSynchronize(d.stream);
+            pic_free(d.ctx, m->pic);
+            m->pic = -1;
+        }
+        if (m->pic < 0) {
+            m->pic = pic_alloc(d.ctx, w, h, true);
+            if (m->pic < 0) die("reference mirror");
+        }
+        m->host = ref;
+        m->w = w;
+        m->h = h;
+        m->stride = stride;
+    }
+    if (stale) {
+        m->shadow.resize(static_cast<size_t>(w) * h);
+        for (int y = 0; y < h; ++y)
+            memcpy(m->shadow.data() + static_cast<size_t>(y) * w, ref + static_cast<size_t>(y) * stride, w);
+        if (tcme_pic_upload(d.ctx, m->pic, m->shadow.data(), w, d.stream) != TCME_OK) die("upload");
+        d.uploads++;
+    }
+    m->last_use = ++d.tick;
+    return m->pic;
+}
+
+}  // namespace
+
+TCME_API tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, int ref_h,
+                               const uint8_t *orig, int orig_stride, int center_x, int center_y,
+                               int blk_size, int search_range, int32_t *best_sad_out)
+{
+    DropIn &d = g_drop;
+    std::lock_guard<std::mutex> g(d.mu);
+    if (!d.ctx) dropin_init();
+    if (blk_size != 8 && blk_size != 16 && blk_size != 32 && blk_size != 64) {
+        fail(TCME_E_BLK, "block size %d (supported: 8, 16, 32, 64)", blk_size);
+        die("tc_motion_est");
+    }
+    if (ref_w <= 0 || ref_h <= 0 || ref_stride < ref_w) {
+        fail(TCME_E_ARG, "reference plane %dx%d stride %d", ref_w, ref_h, ref_stride);
+        die("tc_motion_est");
+    }
+    const int n = blk_size;
+    // Reachable window: two large-hex passes (16*scale), three small-hex
+    // passes (12), four diamond passes (4), the quarter-pel floor (1) and the
+    // six-tap apron (2 left/up, 3 right/down) - margin 16*scale + 24.
+    const int scale = search_range / 4 > 1 ? search_range / 4 : 1;
+    const int reach = 16 * scale + 24;
+    int cx = center_x < 0 ? 0 : (center_x > ref_w - n ? ref_w - n : center_x);
+    int cy = center_y < 0 ? 0 : (center_y > ref_h - n ? ref_h - n : center_y);
+    const int x0 = std::max(0, cx - reach), y0 = std::max(0, cy - reach);
+    const int x1 = std::min(ref_w - 1, cx + n - 1 + reach), y1 = std::min(ref_h - 1, cy + n - 1 + reach);
+    if (hipSetDevice(d.ctx->device) != hipSuccess) die("hipSetDevice");
+    const int rpic = (x0 <= x1 && y0 <= y1) ? mirror_for(ref, ref_stride, ref_w, ref_h, x0, y0, x1, y1)
+                                            : mirror_for(ref, ref_stride, ref_w, ref_h, 0, 0, -1, -1);
+
+    tcme_call call = {rpic, d.cur_pic, 0, 0, center_x, center_y, n, search_range};
+    memcpy(d.h_stage, &call, sizeof call);
+    uint8_t *blk = d.h_stage + 256;
+    for (int r = 0; r < n; ++r) memcpy(blk + r * n, orig + static_cast<size_t>(r) * orig_stride, n);
+    if (hipMemcpyAsync(d.d_call, d.h_stage, sizeof call, hipMemcpyHostToDevice, d.stream) != hipSuccess ||
+        hipMemcpy2DAsync(d.d_blk, d.blk_pitch, blk, n, n, n, hipMemcpyHostToDevice, d.stream) != hipSuccess)
+        die("copy block");
+    if (tcme::launch_search(d.ctx->d_table, d.ctx->max_pics, d.d_call, 1, d.d_res, nullptr, d.stream) != hipSuccess)
+        die("launch");
+    if (hipMemcpyAsync(d.h_res, d.d_res, sizeof(tcme_result), hipMemcpyDeviceToHost, d.stream) != hipSuccess ||
+        hipStreamSynchronize(d.stream) != hipSuccess)
+        die("result");
+    if (d.h_res->status != 0) {
+        fail(d.h_res->status, "kernel rejected the call (status %d)", d.h_res->status);
+        die("tc_motion_est");
+    }
+    d.calls++;
+    if (best_sad_out) *best_sad_out = d.h_res->sad;
+    tcme_mv mv = {d.h_res->mv_x, d.h_res->mv_y};
+    return mv;
+}

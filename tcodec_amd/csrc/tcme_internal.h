@@ -1,0 +1,30 @@
+// Internal interface between the host runtime (tcme_api.hip) and the gfx950
+// kernels (tcme_kernels.hip).  Not installed; the public ABI is include/tcme.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "tcme.h"
+
+namespace tcme {
+
+// Device view of one picture.  The luma plane has `pitch` bytes per row and at
+// least kTailSlack readable bytes after the last row, so a 4-byte read at any
+// in-frame column may fetch the whole aligned 8-byte pair that covers it.
+// hh/vv/cc are the raw (unclipped) six-tap half-sample planes, int16, `hpitch`
+// elements per row: hh(x,y) between (x,y) and (x+1,y); vv(x,y) between (x,y)
+// and (x,y+1); cc(x,y) the centre of the four.  Entries whose taps would leave
+// the frame hold 0 and are never read by the search or the predictor.
+struct PicDev {
+    const uint8_t *y;
+    const int16_t *hh, *vv, *cc;
+    int32_t w, h, pitch, hpitch;
+};
+
+constexpr int kTailSlack = 256;
+constexpr int kMaxBlk = 64;
+
+hipError_t launch_halfpel(const PicDev &p, hipStream_t s);
+hipError_t launch_search(const PicDev *table, int npics, const tcme_call *calls, int n,
+                         tcme_result *res, tcme_stats *stats, hipStream_t s);
+
+}  // namespace tcme

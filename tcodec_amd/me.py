@@ -1,0 +1,189 @@
+"""Host-side mirror of TCodec's motion-estimation interface over libtcme.so.
+
+The reference exposes one C function for this path,
+``tc_motion_est(ref, ref_stride, ref_w, ref_h, orig, orig_stride, center_x,
+center_y, blk_size, search_range, &best_sad)`` (reference
+include/tcodec_common.h:454-459, src/motion.c:384-485).  ``motion_est`` below
+calls the library's drop-in symbol with the same arguments and meaning.  The
+batch class ``Engine`` drives the ``tcme_*`` entry points: device-resident
+pictures and many calls per launch.
+
+Nothing here computes motion estimation: every result comes from the gfx950
+kernels.  Loading fails loudly when the library is missing or no GPU exists.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libtcme.so")
+ROOT = os.path.dirname(HERE)
+
+# Every symbol include/tcme.h declares.
+EXPORTS = (
+    "tc_motion_est", "tcme_last_error", "tcme_version", "tcme_create", "tcme_destroy",
+    "tcme_pic_create", "tcme_pic_destroy", "tcme_pic_upload", "tcme_pic_upload_device",
+    "tcme_pic_luma", "tcme_pic_finish", "tcme_search", "tcme_search_host",
+)
+
+CALL_DTYPE = np.dtype([("ref_pic", "<i4"), ("cur_pic", "<i4"), ("px", "<i4"), ("py", "<i4"),
+                       ("cx", "<i4"), ("cy", "<i4"), ("blk", "<i4"), ("sr", "<i4")])
+RESULT_DTYPE = np.dtype([("mv_x", "<i4"), ("mv_y", "<i4"), ("sad", "<i4"), ("status", "<i4")])
+STATS_DTYPE = np.dtype([("int_evals", "<i4"), ("q6_evals", "<i4"), ("qbil_evals", "<i4"),
+                        ("int_oob", "<i4")])
+
+u8p = C.POINTER(C.c_uint8)
+i32p = C.POINTER(C.c_int32)
+
+
+class MV(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32)]
+
+
+class TcmeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def build() -> None:
+    """Compile libtcme.so for gfx950 (hipcc; runs without a GPU)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "csrc")], check=True)
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load the library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise TcmeError(f"{path} is not built: run __graft_entry__.build()")
+    L = C.CDLL(path)
+    L.tc_motion_est.restype = MV
+    L.tc_motion_est.argtypes = [u8p, C.c_int, C.c_int, C.c_int, u8p, C.c_int, C.c_int, C.c_int,
+                                C.c_int, C.c_int, i32p]
+    L.tcme_last_error.restype = C.c_char_p
+    L.tcme_version.restype = C.c_char_p
+    L.tcme_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int]
+    L.tcme_destroy.argtypes = [C.c_void_p]
+    L.tcme_destroy.restype = None
+    L.tcme_pic_create.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.tcme_pic_destroy.argtypes = [C.c_void_p, C.c_int]
+    L.tcme_pic_upload.argtypes = [C.c_void_p, C.c_int, u8p, C.c_int, C.c_void_p]
+    L.tcme_pic_upload_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+    L.tcme_pic_luma.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), i32p]
+    L.tcme_pic_finish.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.tcme_search.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                              C.c_void_p]
+    L.tcme_search_host.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise TcmeError(f"tcme error {rc}: {_lib.tcme_last_error().decode()}")
+    return rc
+
+
+def motion_est(ref: np.ndarray, ref_stride: int, ref_w: int, ref_h: int, orig: np.ndarray,
+               orig_off: int, orig_stride: int, center_x: int, center_y: int, blk_size: int,
+               search_range: int):
+    """The reference call through the drop-in symbol.
+
+    ``ref`` and ``orig`` are uint8 buffers (any shape, C-contiguous); ``orig_off``
+    is the byte offset of the block's top-left sample inside ``orig``.
+    Returns ``((mv_x, mv_y), sad)`` with mv in absolute quarter-pel units.
+    """
+    L = load()
+    ref = np.ascontiguousarray(ref, np.uint8)
+    orig = np.ascontiguousarray(orig, np.uint8)
+    sad = C.c_int32()
+    mv = L.tc_motion_est(ref.ctypes.data_as(u8p), ref_stride, ref_w, ref_h,
+                         C.cast(orig.ctypes.data + orig_off, u8p), orig_stride, center_x,
+                         center_y, blk_size, search_range, C.byref(sad))
+    return (mv.x, mv.y), sad.value
+
+
+class Engine:
+    """A tcme context: device pictures and batched searches."""
+
+    def __init__(self, device: int = 0, max_pics: int = 1024):
+        self.L = load()
+        h = C.c_void_p()
+        _check(self.L.tcme_create(C.byref(h), device, max_pics))
+        self.h = h
+        self.pics: dict[int, tuple[int, int]] = {}
+
+    def close(self) -> None:
+        if self.h:
+            self.L.tcme_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def pic(self, plane: np.ndarray, stream: int = 0) -> int:
+        """New device picture holding a 2-D uint8 plane (uploaded + half-sample planes)."""
+        plane = np.ascontiguousarray(plane, np.uint8)
+        hh, ww = plane.shape
+        pid = _check(self.L.tcme_pic_create(self.h, ww, hh))
+        self.pics[pid] = (ww, hh)
+        self.upload(pid, plane, stream)
+        return pid
+
+    def new_pic(self, w: int, h: int) -> int:
+        pid = _check(self.L.tcme_pic_create(self.h, w, h))
+        self.pics[pid] = (w, h)
+        return pid
+
+    def upload(self, pid: int, plane: np.ndarray, stream: int = 0) -> None:
+        plane = np.ascontiguousarray(plane, np.uint8)
+        _check(self.L.tcme_pic_upload(self.h, pid, plane.ctypes.data_as(u8p), plane.strides[0],
+                                      C.c_void_p(stream)))
+
+    def upload_device(self, pid: int, dev_ptr: int, stride: int, stream: int = 0) -> None:
+        _check(self.L.tcme_pic_upload_device(self.h, pid, C.c_void_p(dev_ptr), stride,
+                                             C.c_void_p(stream)))
+
+    def free(self, pid: int) -> None:
+        _check(self.L.tcme_pic_destroy(self.h, pid))
+        self.pics.pop(pid, None)
+
+    def search(self, calls: np.ndarray, stats: bool = False, stream: int = 0):
+        """Host call list (CALL_DTYPE or int32 [N,8]) -> results (and stats)."""
+        calls = np.ascontiguousarray(calls)
+        if calls.dtype != CALL_DTYPE:
+            calls = np.ascontiguousarray(calls, np.int32).reshape(-1, 8).view(CALL_DTYPE).ravel()
+        n = calls.shape[0]
+        res = np.zeros(n, RESULT_DTYPE)
+        st = np.zeros(n, STATS_DTYPE) if stats else None
+        _check(self.L.tcme_search_host(self.h, calls.ctypes.data, n, res.ctypes.data,
+                                       st.ctypes.data if stats else None, C.c_void_p(stream)))
+        return (res, st) if stats else res
+
+    def search_device(self, calls_ptr: int, n: int, res_ptr: int, stats_ptr: int | None = None,
+                      stream: int = 0) -> None:
+        """Device call list -> device results; asynchronous on ``stream``."""
+        _check(self.L.tcme_search(self.h, C.c_void_p(calls_ptr), n, C.c_void_p(res_ptr),
+                                  C.c_void_p(stats_ptr) if stats_ptr else None,
+                                  C.c_void_p(stream)))
+
+
+def make_calls(ref_pic, cur_pic, px, py, cx, cy, blk, sr) -> np.ndarray:
+    """Vectorised CALL_DTYPE constructor (scalars broadcast)."""
+    cols = np.broadcast_arrays(*[np.asarray(v, np.int32) for v in
+                                 (ref_pic, cur_pic, px, py, cx, cy, blk, sr)])
+    out = np.zeros(cols[0].size, CALL_DTYPE)
+    for name, col in zip(CALL_DTYPE.names, cols):
+        out[name] = col.ravel()
+    return out

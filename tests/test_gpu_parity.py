@@ -1,0 +1,178 @@
+"""GPU parity: the gfx950 search against the oracle, call for call, bit-exact.
+
+Every test calls through the C ABI of libtcme.so (batch API via ctypes, or the
+drop-in tc_motion_est symbol)."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import golden_traces, moving_pair, random_calls
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_batch(orc, ref, cur, calls, stats=False):
+    arr = np.stack([calls["px"], calls["py"], calls["cx"], calls["cy"], calls["blk"],
+                    calls["sr"]], 1)
+    return orc.motion_est_batch(ref, cur, arr, want_stats=stats)
+
+
+def compare(res, want, calls, label):
+    got = np.stack([res["mv_x"], res["mv_y"], res["sad"]], 1)
+    bad = np.nonzero((got != want).any(1) | (res["status"] != 0))[0]
+    assert bad.size == 0, (f"{label}: {bad.size}/{len(calls)} differ; first call "
+                           f"{calls[bad[0]]} got {got[bad[0]]} want {want[bad[0]]} "
+                           f"status {res['status'][bad[0]]}")
+
+
+@pytest.mark.parametrize("trace", golden_traces(), ids=lambda t: t[0])
+def test_gpu_replays_golden_trace(engine, trace):
+    from tcodec_amd import me
+    name, refs, curs, calls = trace
+    log(f"golden {name}: {len(calls)} calls")
+    rp = {p: engine.pic(pl) for p, pl in refs.items()}
+    cp = {i: engine.pic(pl) for i, pl in curs.items()}
+    try:
+        c = me.make_calls([rp[int(v)] for v in calls[:, 3]], [cp[int(v)] for v in calls[:, 0]],
+                          calls[:, 1], calls[:, 2], calls[:, 4], calls[:, 5], calls[:, 6],
+                          calls[:, 7])
+        res = engine.search(c)
+        compare(res, calls[:, 10:13], c, name)
+    finally:
+        for p in list(rp.values()) + list(cp.values()):
+            engine.free(p)
+
+
+@pytest.mark.parametrize("shape", [(96, 128), (64, 80), (72, 200), (128, 64)])
+def test_gpu_random_calls_vs_oracle(engine, orc, shape):
+    from tcodec_amd import me
+    h, w = shape
+    rng = np.random.default_rng(h * 1000 + w)
+    log(f"random calls {w}x{h}")
+    for trial in range(4):
+        ref, cur = moving_pair(h, w, trial + h, shift=tuple(int(v) for v in rng.integers(-9, 10, 2)))
+        rp, cp = engine.pic(ref), engine.pic(cur)
+        arr = random_calls(rng, w, h, 600)
+        calls = me.make_calls(rp, cp, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], arr[:, 4],
+                              arr[:, 5])
+        res, st = engine.search(calls, stats=True)
+        want, wst = oracle_batch(orc, ref, cur, calls, stats=True)
+        compare(res, want, calls, f"{w}x{h} trial {trial}")
+        # candidate counts follow the reference's sequential order exactly
+        assert np.array_equal(st["int_evals"], wst[:, 0])
+        assert np.array_equal(st["int_oob"], wst[:, 1])
+        assert np.array_equal(st["q6_evals"], wst[:, 2])
+        assert np.array_equal(st["qbil_evals"], wst[:, 3])
+        engine.free(rp)
+        engine.free(cp)
+
+
+def test_gpu_every_node_of_a_synthetic_frame(engine, orc):
+    """The throughput call set on a scene frame pair: all quadtree nodes,
+    both ranges, with a moving texture so the walk travels."""
+    from tcodec_amd import synth, workload
+    log("scene 320x240 all nodes")
+    fr = synth.scene_video(320, 240, 3, seed=3, rects=8, pan=(0.75, 0.25))
+    ref, cur = fr[0][0], fr[2][0]
+    rp, cp = engine.pic(ref), engine.pic(cur)
+    nodes = workload.qt_nodes(320, 240)
+    for sr in (workload.SR_P, workload.SR_B):
+        calls = workload.frame_calls(nodes, [rp], cp, sr)
+        res = engine.search(calls)
+        compare(res, oracle_batch(orc, ref, cur, calls), calls, f"sr {sr}")
+    engine.free(rp)
+    engine.free(cp)
+
+
+def test_gpu_edge_frames(engine, orc):
+    """Frames at and below the block size and the 6-tap margins; centres far
+    outside; the bilinear and skipped quarter-pel paths."""
+    from tcodec_amd import me
+    rng = np.random.default_rng(77)
+    log("edge frames")
+    for (h, w) in [(8, 8), (9, 13), (16, 12), (20, 40), (33, 17), (64, 64), (66, 70)]:
+        ref = rng.integers(0, 256, (h, w)).astype(np.uint8)
+        ref[:, : w // 2] = np.clip(ref[:, : w // 2] // 4 + 60, 0, 255)
+        cur = rng.integers(0, 256, (64, 64)).astype(np.uint8)
+        cur[8:40, 8:40] = ref[: min(h, 32), : min(w, 32)].mean()
+        rp, cp = engine.pic(ref), engine.pic(cur)
+        arr = []
+        for n in (8, 16, 32, 64):
+            for cx, cy in [(0, 0), (-5, 3), (w, h), (2, 1), (-100, 300)]:
+                for sr in (1, 8, 24, 32, 64):
+                    arr.append((0, 0, cx, cy, n, sr))
+        arr = np.array(arr, np.int32)
+        calls = me.make_calls(rp, cp, arr[:, 0], arr[:, 1], arr[:, 2], arr[:, 3], arr[:, 4],
+                              arr[:, 5])
+        res = engine.search(calls)
+        want = oracle_batch(orc, ref, cur, calls)
+        compare(res, want, calls, f"edge {w}x{h}")
+        engine.free(rp)
+        engine.free(cp)
+
+
+def test_gpu_rejects_bad_descriptors(engine):
+    from tcodec_amd import me
+    ref = np.zeros((32, 32), np.uint8)
+    rp = engine.pic(ref)
+    calls = me.make_calls([rp, rp, 999, rp, rp], [rp, rp, rp, rp, -1],
+                          [0, 24, 0, 0, 0], [0, 0, 0, 0, 0], 0, 0, [8, 16, 8, 12, 8], 24)
+    res = engine.search(calls)
+    assert res["status"].tolist() == [0, -1, -4, -5, -4]
+    engine.free(rp)
+
+
+def test_gpu_full_hd_sample(engine, orc):
+    """A sample of the 1080p B workload checked exactly (the oracle is too
+    slow for every call), plus the whole frame's call set run for status and
+    a size-independent property: every result is at least as good as its
+    clipped centre and lies on the frame."""
+    from tcodec_amd import synth, workload
+    log("1080p sample")
+    sc = synth.Scene(1920, 1080, 7, 48, (0.75, 0.25))
+    ref, cur = sc.frame(0)[0], sc.frame(2)[0]
+    rp, cp = engine.pic(ref), engine.pic(cur)
+    nodes = workload.qt_nodes(1920, 1080)
+    calls = workload.frame_calls(nodes, [rp], cp, workload.SR_B)
+    res = engine.search(calls)
+    assert (res["status"] == 0).all()
+    rng = np.random.default_rng(1)
+    pick = np.sort(rng.choice(len(calls), 3000, replace=False))
+    want = oracle_batch(orc, ref, cur, calls[pick])
+    compare(res[pick], want, calls[pick], "1080p sample")
+    n = calls["blk"]
+    assert ((res["mv_x"] >> 2) >= -1).all() and ((res["mv_x"] >> 2) <= 1920 - n).all()
+    assert ((res["mv_y"] >> 2) >= -1).all() and ((res["mv_y"] >> 2) <= 1080 - n).all()
+    engine.free(rp)
+    engine.free(cp)
+
+
+def test_dropin_symbol_matches_oracle(orc):
+    """tc_motion_est through the drop-in symbol: strided planes, v1-style
+    8x8 stack blocks (stride 8), and a reference buffer reused with new
+    content between calls (the mirror must notice)."""
+    from tcodec_amd import me
+    rng = np.random.default_rng(21)
+    log("drop-in")
+    h, w, stride = 96, 120, 128
+    for trial in range(3):
+        ref, cur = moving_pair(h, w, trial, shift=(2 - trial, 3))
+        rbuf = np.zeros((h, stride), np.uint8)
+        rbuf[:, :w] = ref
+        for px, py, cx, cy, n, sr in random_calls(rng, w, h, 80):
+            (mx, my), sad = me.motion_est(rbuf, stride, w, h, cur, py * w + px, w, cx, cy, n, sr)
+            assert (mx, my, sad) == orc.motion_est(ref, cur, px, py, cx, cy, n, sr)
+            blk = np.ascontiguousarray(cur[py:py + 8, px:px + 8])
+            (mx, my), sad = me.motion_est(rbuf, stride, w, h, blk, 0, 8, cx, cy, 8, sr)
+            assert (mx, my, sad) == orc.motion_est(ref, cur, px, py, cx, cy, 8, sr)
+        # same buffer, new content: next calls must see it
+        rbuf[:, :w] = np.roll(ref, 5, 1)
+        new = np.ascontiguousarray(rbuf[:, :w])
+        (mx, my), sad = me.motion_est(rbuf, stride, w, h, cur, 0, w, 40, 40, 16, 24)
+        assert (mx, my, sad) == orc.motion_est(new, cur, 0, 0, 40, 40, 16, 24)

@@ -1,0 +1,271 @@
+"""Pins the oracle (oracle/tcme_oracle.c) before it is trusted as the checker.
+
+* against the reference compiled from its own sources (oracle/_ref/libtcref.so)
+* against golden call traces recorded from the reference encoder
+* against closed forms and brute force where the mathematics fixes the answer
+"""
+import numpy as np
+import pytest
+
+from helpers import golden_traces, moving_pair, random_calls, smooth_plane
+
+
+# ---- golden traces (committed data; needs no reference build) -------------
+
+@pytest.mark.parametrize("trace", golden_traces(), ids=lambda t: t[0])
+def test_oracle_replays_golden_trace(orc, trace):
+    name, refs, curs, calls = trace
+    bad = []
+    for c in calls:
+        cid, px, py, rp, cx, cy, blk, sr, w, h, mx, my, sad = (int(v) for v in c[:13])
+        assert refs[rp].shape == (h, w)
+        got = orc.motion_est(refs[rp], curs[cid], px, py, cx, cy, blk, sr)
+        if got != (mx, my, sad):
+            bad.append((c.tolist(), got))
+    assert not bad, f"{len(bad)} of {len(calls)} calls differ, first: {bad[:3]}"
+
+
+def test_golden_traces_cover_the_call_sites():
+    """The traces hold every block size, both search ranges the v2 medium
+    preset uses, P multiref (two reference POCs per current picture) and
+    B forward/backward (a reference on each side of the current picture)."""
+    seen_blk, seen_sr = set(), set()
+    multi = False
+    for name, refs, curs, calls in golden_traces():
+        seen_blk |= set(calls[:, 6].tolist())
+        seen_sr |= set(calls[:, 7].tolist())
+        per_cur = {}
+        for c in calls:
+            per_cur.setdefault(int(c[0]), set()).add(int(c[3]))
+        multi |= any(len(v) >= 2 for v in per_cur.values())
+    assert {8, 16, 32, 64} <= seen_blk
+    assert {24, 32} <= seen_sr
+    assert multi
+
+
+# ---- against the compiled reference ---------------------------------------
+
+def test_motion_est_matches_reference_random(orc, ref_lib):
+    rng = np.random.default_rng(3)
+    bad = 0
+    total = 0
+    for trial in range(60):
+        h, w = [(96, 128), (64, 80), (40, 72), (128, 96)][trial % 4]
+        ref, cur = moving_pair(h, w, trial, shift=tuple(rng.integers(-9, 10, 2)))
+        for px, py, cx, cy, n, sr in random_calls(rng, w, h, 25):
+            a = orc.motion_est(ref, cur, px, py, cx, cy, n, sr)
+            b = ref_lib.motion_est(ref, cur, px, py, cx, cy, n, sr)
+            total += 1
+            bad += a != b
+    assert total > 1000 and bad == 0
+
+
+def test_motion_est_matches_reference_odd_sizes(orc, ref_lib):
+    """Frames narrower/shorter than the block or the 6-tap margin, block
+    sizes the encoders never use, padded strides."""
+    rng = np.random.default_rng(5)
+    for (h, w) in [(8, 8), (9, 13), (16, 12), (20, 40), (33, 17)]:
+        big = rng.integers(0, 256, (h, w + 11)).astype(np.uint8)
+        ref = big[:, :w]
+        cur = np.ascontiguousarray(rng.integers(0, 256, (64, 64)).astype(np.uint8))
+        for n in (4, 8, 12, 16, 32):
+            for cx, cy in [(0, 0), (-5, 3), (w, h), (2, 1)]:
+                for sr in (1, 8, 24):
+                    a = orc.motion_est(ref, cur, 0, 0, cx, cy, n, sr)
+                    b = ref_lib.motion_est(ref, cur, 0, 0, cx, cy, n, sr)
+                    assert a == b, (h, w, n, cx, cy, sr, a, b)
+
+
+def test_inter_predict_matches_reference(orc, ref_lib):
+    rng = np.random.default_rng(9)
+    for trial in range(1500):
+        h, w = int(rng.integers(6, 48)), int(rng.integers(6, 48))
+        ref = rng.integers(0, 256, (h, w)).astype(np.uint8)
+        n = int(rng.choice([4, 8, 16]))
+        mvx = int(rng.integers(-40, 4 * w + 40))
+        mvy = int(rng.integers(-40, 4 * h + 40))
+        assert np.array_equal(orc.inter_predict(ref, mvx, mvy, n),
+                              ref_lib.inter_predict(ref, mvx, mvy, n)), (h, w, n, mvx, mvy)
+        assert np.array_equal(orc.inter_predict_chroma(ref, mvx, mvy, n),
+                              ref_lib.inter_predict_chroma(ref, mvx, mvy, n))
+
+
+def test_mvp_matches_reference(orc, ref_lib):
+    rng = np.random.default_rng(4)
+    for trial in range(4000):
+        if trial % 3 == 0:
+            dx = rng.integers(-2, 3, 64)
+            dy = rng.integers(-2, 3, 64)
+        else:
+            dx = rng.integers(-400, 400, 64)
+            dy = rng.integers(-400, 400, 64)
+        intra = (rng.random(64) < [0.0, 0.3, 0.9][trial % 3]).astype(np.uint8)
+        cx, cy = int(rng.integers(0, 8)), int(rng.integers(0, 8))
+        assert orc.mvp(cx, cy, dx, dy, intra) == ref_lib.mvp(cx, cy, dx, dy, intra)
+
+
+def test_global_mv_matches_reference(orc, ref_lib):
+    rng = np.random.default_rng(8)
+    for trial in range(12):
+        s = smooth_plane(120, 176, trial + 100, cell=12)
+        ref = np.clip(s, 0, 255).astype(np.uint8)
+        sh = tuple(int(v) for v in rng.integers(-36, 37, 2))
+        cur = np.clip(np.roll(s, sh, (0, 1)) + rng.integers(-3, 4, s.shape), 0, 255).astype(np.uint8)
+        assert orc.global_mv(cur, ref) == ref_lib.global_mv(cur, ref)
+
+
+# ---- closed forms and brute force (no reference needed) -------------------
+
+def test_half_samples_closed_forms(orc):
+    """Taps [1,-5,20,20,-5,1]/32 with +16 rounding: constants reproduce, a
+    linear ramp a*x gives its midpoint, a unit impulse gives the taps/32
+    rounded, and the centre sample of a separable ramp is the midpoint."""
+    flat = np.full((16, 16), 77, np.uint8)
+    assert orc.lib.orc_half_h(flat.ctypes.data_as(__import__("ctypes").POINTER(__import__("ctypes").c_uint8)), 16, 6, 6) == 77
+    import oracle
+    p = oracle.ptr
+    for a in (1, 2, 5, 15):
+        ramp = np.tile((np.arange(16) * a).astype(np.uint8), (16, 1))
+        for x in range(2, 13):
+            # exact midpoint a*(x+0.5); the six-tap is exact on linear input
+            assert orc.lib.orc_half_h(p(ramp), 16, x, 5) == (32 * a * x + 16 * a + 16) >> 5
+            assert orc.lib.orc_half_v(p(np.ascontiguousarray(ramp.T)), 16, 5, x) == \
+                (32 * a * x + 16 * a + 16) >> 5
+    imp = np.zeros((16, 16), np.uint8)
+    imp[8, 8] = 255
+    taps = [1, -5, 20, 20, -5, 1]
+    for k, t in enumerate(taps):
+        x = 8 + 2 - k  # tap k of position x reads x-2+k
+        assert orc.lib.orc_half_h(p(imp), 16, x, 8) == (255 * t + 16) >> 5
+    ramp2 = (np.arange(16)[:, None] * 3 + np.arange(16)[None, :] * 5).astype(np.uint8)
+    # rows first: h(r) = round(3r + 5*6.5) = 3r + 33 (exact on a ramp);
+    # then the vertical pass over h(r): round(h(7.5)) with the same rule
+    assert orc.lib.orc_half_hv(p(ramp2), 16, 6, 7) == (32 * (3 * 7 + 33) + 16 * 3 + 16) >> 5
+
+
+def test_half_sample_bounds_brute_force(orc):
+    """Extremes of the raw half samples over worst-case inputs:
+    h, v in [-80, 335]; centre in [-210, 465] (int16 planes are safe)."""
+    import oracle
+    p = oracle.ptr
+    worst = []
+    for pat in ([0, 255, 0, 0, 255, 0], [255, 0, 255, 255, 0, 255]):
+        row = np.array(pat + [0] * 10, np.uint8).reshape(1, 16).repeat(16, 0)
+        worst.append(orc.lib.orc_half_h(p(row), 16, 2, 3))
+    assert min(worst) == -80 and max(worst) == 335
+    # centre: rows alternate between the two extreme patterns
+    lo = np.zeros((16, 16), np.uint8)
+    hi = np.zeros((16, 16), np.uint8)
+    a = np.array([0, 255, 0, 0, 255, 0] + [0] * 10, np.uint8)
+    b = np.array([255, 0, 255, 255, 0, 255] + [0] * 10, np.uint8)
+    for r, sgn in enumerate([1, -1, 1, 1, -1, 1]):
+        hi[r] = b if sgn > 0 else a   # row half sample 335 under a + tap, -80 under a - tap
+        lo[r] = a if sgn > 0 else b
+    assert orc.lib.orc_half_hv(p(hi), 16, 2, 2) == 465
+    assert orc.lib.orc_half_hv(p(lo), 16, 2, 2) == -210
+
+
+def test_sad_brute_force(orc):
+    import oracle
+    rng = np.random.default_rng(2)
+    for n in (1, 4, 8, 16, 64):
+        a = rng.integers(0, 256, (n, n + 3)).astype(np.uint8)
+        b = rng.integers(0, 256, (n, n + 5)).astype(np.uint8)
+        want = int(np.abs(a[:, :n].astype(int) - b[:, :n].astype(int)).sum())
+        assert orc.lib.orc_sad(oracle.ptr(a), a.strides[0], oracle.ptr(b), b.strides[0], n) == want
+
+
+def test_qpel_phases_reduce_to_integer_and_half(orc):
+    """Phase (0,0) is the integer sample; (2,0), (0,2), (2,2) are the clipped
+    raw half samples; (1,0) is the rounded mean of integer and half."""
+    import oracle
+    p = oracle.ptr
+    rng = np.random.default_rng(1)
+    img = rng.integers(0, 256, (20, 20)).astype(np.uint8)
+    for (x, y) in [(4, 4), (7, 9), (12, 5)]:
+        clip = lambda v: max(0, min(255, v))
+        hb = orc.lib.orc_half_h(p(img), 20, x, y)
+        vc = orc.lib.orc_half_v(p(img), 20, x, y)
+        cd = orc.lib.orc_half_hv(p(img), 20, x, y)
+        q = lambda dx, dy: orc.lib.orc_qpel6(p(img), 20, x, y, dx, dy)
+        assert q(0, 0) == img[y, x]
+        assert q(2, 0) == clip(hb)
+        assert q(0, 2) == clip(vc)
+        assert q(2, 2) == clip(cd)
+        assert q(1, 0) == clip((int(img[y, x]) + hb + 1) >> 1)
+        assert q(3, 0) == clip((hb + int(img[y, x + 1]) + 1) >> 1)
+
+
+def test_search_invariants(orc):
+    """Properties the search must satisfy whatever its walk: the result lies
+    in the frame; its SAD is the block cost at the returned position; it is
+    never worse than the clipped centre; on an exact copy the centre wins."""
+    import oracle
+    rng = np.random.default_rng(12)
+    for trial in range(30):
+        ref, cur = moving_pair(80, 96, trial, shift=tuple(rng.integers(-6, 7, 2)))
+        for px, py, cx, cy, n, sr in random_calls(rng, 96, 80, 10, sizes=(8, 16, 32)):
+            (mx, my, sad), st = orc.motion_est(ref, cur, px, py, cx, cy, n, sr, stats=True)
+            ccx = min(max(cx, 0), 96 - n)
+            ccy = min(max(cy, 0), 80 - n)
+            blk = np.ascontiguousarray(cur[py:py + n, px:px + n])
+            c0 = int(np.abs(ref[ccy:ccy + n, ccx:ccx + n].astype(int) - blk.astype(int)).sum())
+            assert sad <= c0
+            fx, fy = mx >> 2, my >> 2
+            assert 0 <= fx <= 96 - n and 0 <= fy <= 80 - n or (mx & 3 or my & 3)
+            if (mx & 3) == 0 and (my & 3) == 0:
+                assert sad == int(np.abs(ref[fy:fy + n, fx:fx + n].astype(int) - blk.astype(int)).sum())
+            elif fx >= 2 and fy >= 2 and fx + n + 2 < 96 and fy + n + 2 < 80:
+                assert sad == orc.lib.orc_sad_qpel6(oracle.ptr(ref), 96, mx, my, oracle.ptr(blk), n, n)
+            else:
+                assert sad == orc.lib.orc_sad_subpel(oracle.ptr(ref), 96, mx, my, oracle.ptr(blk), n, n)
+            assert st.int_evals >= 1
+    ref, _ = moving_pair(64, 64, 1)
+    for n in (8, 16, 32):
+        assert orc.motion_est(ref, ref, 16, 16, 16, 16, n, 24) == (64, 64, 0)
+
+
+def test_planted_mistake_is_caught(orc, ref_lib):
+    """A plausible slip in a restatement - ties resolved to the later
+    candidate (<= instead of <) - must fail the reference comparison.  The
+    slip is planted in a Python re-walk built on the oracle's own SAD."""
+    import oracle
+
+    def walk(ref, cur, px, py, cx, cy, n, sr, le):
+        h, w = ref.shape
+        blk = np.ascontiguousarray(cur[py:py + n, px:px + n])
+
+        def cost(x, y):
+            if x < 0 or y < 0 or x + n > w or y + n > h:
+                return 0x7FFFFFFF
+            return int(np.abs(ref[y:y + n, x:x + n].astype(int) - blk.astype(int)).sum())
+
+        better = (lambda s, b: s <= b) if le else (lambda s, b: s < b)
+        cx = min(max(cx, 0), w - n)
+        cy = min(max(cy, 0), h - n)
+        bx, by, best = cx, cy, cost(cx, cy)
+        pats = [([(-2, -4), (2, -4), (-4, 0), (4, 0), (-2, 4), (2, 4)], max(1, sr // 4), 2),
+                ([(-1, -2), (1, -2), (-2, 0), (2, 0), (-1, 2), (1, 2)], 1, 3),
+                ([(0, -1), (-1, 0), (1, 0), (0, 1)], 1, 4)]
+        for pat, sc, passes in pats:
+            for _ in range(passes):
+                moved = False
+                for dx, dy in pat:
+                    s = cost(bx + dx * sc, by + dy * sc)
+                    if better(s, best):
+                        best, bx, by, moved = s, bx + dx * sc, by + dy * sc, True
+                if not moved:
+                    break
+        return bx, by, best
+
+    flat = np.full((48, 48), 100, np.uint8)  # every candidate ties
+    rng = np.random.default_rng(0)
+    diffs = 0
+    for _ in range(5):
+        cx, cy = int(rng.integers(8, 24)), int(rng.integers(8, 24))
+        good = walk(flat, flat, 8, 8, cx, cy, 8, 16, le=False)
+        slip = walk(flat, flat, 8, 8, cx, cy, 8, 16, le=True)
+        r = ref_lib.motion_est(flat, flat, 8, 8, cx, cy, 8, 16)
+        assert (good[0] * 4, good[1] * 4) == (r[0], r[1])
+        diffs += (slip[0] * 4, slip[1] * 4) != (r[0], r[1])
+    assert diffs == 5
