@@ -25,6 +25,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 W, H = 1920, 1080
 DISPLAY_FRAMES = 13
 TIMED_EMITS = range(5, 13)
@@ -37,13 +41,13 @@ def build_workload(eng):
     for t in range(DISPLAY_FRAMES):
         pics[t] = eng.pic(scene.frame(t)[0])
     order = workload.gop4_coding_order(DISPLAY_FRAMES)
-    nodes = workload.qt_nodes(W, H)
     frames = []
     for e in TIMED_EMITS:
         poc, kind, dpb = order[e]
         assert len(dpb) == 4, (poc, dpb)
         sr = workload.SR_B if kind == "B" else workload.SR_P
-        frames.append((poc, kind, workload.frame_calls(nodes, [pics[p] for p in dpb], pics[poc], sr)))
+        frames.append((poc, kind, workload.frame_calls_by_ctu(W, H, [pics[p] for p in dpb],
+                                                              pics[poc], sr)))
     return scene, pics, frames
 
 
@@ -92,6 +96,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--mode", choices=("tuned", "plain"), default="tuned",
+                    help="plain: one wavefront per call only; tuned: also the autotuned "
+                         "chunked launch (reported as value)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -106,8 +113,10 @@ def main():
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
     eng = me.Engine(0)
+    log("generating and uploading 13 frames of 1920x1080")
     scene, pics, frames = build_workload(eng)
     torch.cuda.synchronize()
+    log(f"{len(frames)} frames, {sum(len(c) for _, _, c in frames)} calls")
 
     d_calls, d_res, d_stats = [], [], []
     for poc, kind, calls in frames:
@@ -133,20 +142,36 @@ def main():
         st = d_stats[i].cpu().numpy().view(me.STATS_DTYPE).ravel()
         algo.append(workload.algo_bytes(st, calls["blk"]))
         assert (d_res[i][:, 3] == 0).all().item()
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
+    log("statistics pass done")
+    def timed(label):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        evs_all = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                  for _ in frames]
+            step(events=ev)
+            evs_all.append(ev)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        lat_ = np.array([a.elapsed_time(b) for evs in evs_all for a, b in evs])
+        log(f"{label}: {args.steps} steps in {el:.3f} s, frame p50 {np.percentile(lat_, 50):.3f} ms")
+        return el, lat_
 
-    per_frame_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in frames]
-        step(events=ev)
-        per_frame_ms.append(ev)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    lat = np.array([a.elapsed_time(b) for evs in per_frame_ms for a, b in evs])
+    # plain form: one wavefront per call, no LDS staging
+    eng.set_tiling(1, 1, 0, 0)
+    plain_el, plain_lat = timed("plain")
+    if args.mode == "tuned":
+        # tuned form: the autotuner picks the launch shape on frame 0
+        best, best_ms = eng.autotune(d_calls[0].data_ptr(), len(frames[0][2]),
+                                     d_res[0].data_ptr(), reps=2, stream=sptr)
+        log(f"autotune: {best.as_dict()} {best_ms:.3f} ms")
+        elapsed, lat = timed("tuned")
+    else:
+        best = eng.tiling()
+        elapsed, lat = plain_el, plain_lat
     nframes = len(frames) * args.steps
     fps = nframes / elapsed
     kernel_s = lat.mean() / 1e3
@@ -174,6 +199,10 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),
                        "load": "back-to-back per-frame launches, one stream"},
+        "plain": {"value": round(nframes / plain_el, 2),
+                  "p50_ms": round(float(np.percentile(plain_lat, 50)), 3),
+                  "shape": "one wavefront per call, no LDS window"},
+        "tiling": best.as_dict(),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,

@@ -100,6 +100,23 @@ int  tcme_pic_finish(tcme_ctx *ctx, int pic, void *stream);
 int  tcme_search(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,
                  tcme_stats *stats, void *stream);
 
+/* Launch shape of tcme_search: calls per workgroup (a workgroup stages the
+ * union of its calls' reference windows in LDS), wavefronts per workgroup
+ * (1, 2, 4 or 8), window apron in pixels around each clamped centre, and LDS
+ * bytes per workgroup (<= 65536).  Results never depend on it. */
+typedef struct tcme_tiling {
+    int32_t chunk, waves, apron, lds_bytes;
+} tcme_tiling;
+
+int  tcme_set_tiling(tcme_ctx *ctx, const tcme_tiling *t);
+int  tcme_get_tiling(tcme_ctx *ctx, tcme_tiling *t);
+
+/* Times a grid of launch shapes on the caller's call list (device arrays;
+ * results are overwritten with the same answers) and keeps the fastest in
+ * the context.  Synchronises the stream. */
+int  tcme_autotune(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,
+                   int reps, void *stream, tcme_tiling *best, float *best_ms);
+
 /* Host-pointer convenience: copies calls in, searches, copies results out,
  * synchronises the stream. */
 int  tcme_search_host(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,

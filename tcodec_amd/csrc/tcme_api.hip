@@ -1,7 +1,9 @@
 // Host runtime and C ABI of libtcme.so (declared in include/tcme.h).
 //
 // Batch API: a context owns a device picture table; pictures are luma planes
-// with their half-sample planes; tcme_search() launches one wavefront per call.
+// with their half-sample planes; tcme_search() launches the search in the
+// context's launch shape (plain: one wavefront per call; chunked: LDS-staged
+// reference windows shared by a workgroup's calls).
 //
 // Drop-in: tc_motion_est() keeps device mirrors of the reference planes it is
 // handed.  Before each call it compares, byte for byte, the part of the host
@@ -68,6 +70,7 @@ struct tcme_ctx {
     tcme_result *d_res = nullptr;
     tcme_stats *d_stats = nullptr;
     size_t stage_cap = 0;
+    tcme::Tiling tiling;
     std::mutex mu;
 };
 
@@ -260,8 +263,116 @@ TCME_API int tcme_search(tcme_ctx *c, const tcme_call *calls, int n, tcme_result
     if (n < 0 || (n > 0 && (!calls || !results))) return fail(TCME_E_ARG, "bad call list");
     if (n == 0) return TCME_OK;
     if (set_device(c) != TCME_OK) return TCME_E_HIP;
-    HIP_TRY(tcme::launch_search(c->d_table, c->max_pics, calls, n, results, stats,
-                                static_cast<hipStream_t>(stream)));
+    HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, c->tiling, results,
+                                      stats, static_cast<hipStream_t>(stream)));
+    return TCME_OK;
+}
+
+namespace {
+
+int check_tiling(const tcme_tiling *t)
+{
+    if (!t) return fail(TCME_E_ARG, "null tiling");
+    if (t->chunk < 1 || t->chunk > 4096) return fail(TCME_E_ARG, "chunk %d out of [1,4096]", t->chunk);
+    if (t->waves != 1 && t->waves != 2 && t->waves != 4 && t->waves != 8)
+        return fail(TCME_E_ARG, "waves %d not in {1,2,4,8}", t->waves);
+    if (t->apron < 0 || t->apron > 512) return fail(TCME_E_ARG, "apron %d out of [0,512]", t->apron);
+    if (t->lds_bytes < 0 || t->lds_bytes > 64 * 1024)
+        return fail(TCME_E_ARG, "lds_bytes %d out of [0,65536]", t->lds_bytes);
+    return TCME_OK;
+}
+
+tcme::Tiling to_internal(const tcme_tiling &t)
+{
+    tcme::Tiling r;
+    r.chunk = t.chunk;
+    r.waves = t.waves;
+    r.apron = t.apron;
+    r.lds_bytes = t.lds_bytes;
+    return r;
+}
+
+}  // namespace
+
+TCME_API int tcme_set_tiling(tcme_ctx *c, const tcme_tiling *t)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    if (int e = check_tiling(t)) return e;
+    c->tiling = to_internal(*t);
+    return TCME_OK;
+}
+
+TCME_API int tcme_get_tiling(tcme_ctx *c, tcme_tiling *t)
+{
+    if (!c || !t) return fail(TCME_E_ARG, "null argument");
+    t->chunk = c->tiling.chunk;
+    t->waves = c->tiling.waves;
+    t->apron = c->tiling.apron;
+    t->lds_bytes = c->tiling.lds_bytes;
+    return TCME_OK;
+}
+
+// Times every launch shape of a small grid on the caller's call list (device
+// arrays; results are overwritten) and keeps the fastest in the context.
+// Results do not depend on the shape, only time does.
+TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_result *results,
+                           int reps, void *stream, tcme_tiling *best_out, float *best_ms)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    if (n <= 0 || !calls || !results) return fail(TCME_E_ARG, "bad call list");
+    if (reps < 1) reps = 3;
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    static const int chunks[] = {43, 85, 170};
+    static const int waves[] = {2, 4, 8};
+    static const int aprons[] = {24, 40, 56};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    tcme::Tiling best;  // starts at the plain form, which is timed first
+    float best_t = 1e30f;
+    {
+        const tcme::Tiling plain;
+        HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, plain, results,
+                                          nullptr, s));
+        HIP_TRY(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r)
+            HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, plain, results,
+                                              nullptr, s));
+        HIP_TRY(hipEventRecord(e1, s));
+        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipEventElapsedTime(&best_t, e0, e1));
+        best_t /= reps;
+    }
+    for (int ch : chunks)
+        for (int wv : waves)
+            for (int ap : aprons) {
+                tcme::Tiling t;
+                t.chunk = ch;
+                t.waves = wv;
+                t.apron = ap;
+                const int span = 64 * ((ch + 84) / 85);
+                int lds = (span + 2 * ap + 12) * (64 + 2 * ap) + 64;
+                lds = (lds + 1023) / 1024 * 1024;
+                t.lds_bytes = lds > 64 * 1024 ? 64 * 1024 : lds;
+                HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, t, results,
+                                                  nullptr, s));  // warm
+                HIP_TRY(hipEventRecord(e0, s));
+                for (int r = 0; r < reps; ++r)
+                    HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, t,
+                                                      results, nullptr, s));
+                HIP_TRY(hipEventRecord(e1, s));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                ms /= reps;
+                if (ms < best_t) { best_t = ms; best = t; }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->tiling = best;
+    if (best_out) tcme_get_tiling(c, best_out);
+    if (best_ms) *best_ms = best_t;
     return TCME_OK;
 }
 
@@ -288,8 +399,8 @@ TCME_API int tcme_search_host(tcme_ctx *c, const tcme_call *calls, int n, tcme_r
         c->stage_cap = cap;
     }
     HIP_TRY(hipMemcpyAsync(c->d_calls, calls, n * sizeof(tcme_call), hipMemcpyHostToDevice, s));
-    HIP_TRY(tcme::launch_search(c->d_table, c->max_pics, c->d_calls, n, c->d_res,
-                                stats ? c->d_stats : nullptr, s));
+    HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, c->d_calls, n, c->tiling,
+                                      c->d_res, stats ? c->d_stats : nullptr, s));
     HIP_TRY(hipMemcpyAsync(results, c->d_res, n * sizeof(tcme_result), hipMemcpyDeviceToHost, s));
     if (stats)
         HIP_TRY(hipMemcpyAsync(stats, c->d_stats, n * sizeof(tcme_stats), hipMemcpyDeviceToHost, s));

@@ -23,8 +23,22 @@ struct PicDev {
 constexpr int kTailSlack = 256;
 constexpr int kMaxBlk = 64;
 
+// Launch shape of the search (chosen by tcme_autotune or set by the caller):
+// calls per workgroup, wavefronts per workgroup, LDS window apron in pixels
+// and LDS bytes per workgroup.  chunk 1 with no LDS is the plain form, one
+// wavefront per call (me_search_kernel); it is the default.
+struct Tiling {
+    int chunk = 1;
+    int waves = 1;
+    int apron = 0;
+    int lds_bytes = 0;
+};
+
 hipError_t launch_halfpel(const PicDev &p, hipStream_t s);
 hipError_t launch_search(const PicDev *table, int npics, const tcme_call *calls, int n,
                          tcme_result *res, tcme_stats *stats, hipStream_t s);
+hipError_t launch_search_tiled(const PicDev *table, int npics, const tcme_call *calls, int n,
+                               const Tiling &t, tcme_result *res, tcme_stats *stats,
+                               hipStream_t s);
 
 }  // namespace tcme

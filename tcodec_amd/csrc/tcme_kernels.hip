@@ -25,12 +25,31 @@ __device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f)
     return (a + f - 5 * (b + e) + 20 * (c + d) + 16) >> 5;
 }
 
-// Four bytes at any address, from the aligned dword pair that covers them.
-__device__ __forceinline__ uint32_t ld4(const uint8_t *p)
+// Global-memory view of a plane (address space 1), so plane reads compile to
+// global loads rather than flat ones.
+typedef const __attribute__((address_space(1))) uint32_t *gptr32;
+
+__device__ __forceinline__ gptr32 as_global32(const void *p)
 {
-    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
-    const uint32_t *q = reinterpret_cast<const uint32_t *>(a & ~uintptr_t(3));
-    return __builtin_amdgcn_alignbyte(q[1], q[0], static_cast<uint32_t>(a & 3));
+    return (gptr32)(p);
+}
+
+// Four bytes at byte offset `off` of a 4-byte-aligned buffer, assembled from
+// the aligned dword pair that covers them.  Offsets, never integer-cast
+// pointers, so every access keeps its address space.
+__device__ __forceinline__ uint32_t ld4g(gptr32 base, size_t off)
+{
+    gptr32 q = base + (off >> 2);
+    return __builtin_amdgcn_alignbyte(q[1], q[0], static_cast<uint32_t>(off & 3));
+}
+
+// The LDS reference window of the chunked kernel (dynamic shared memory).
+extern __shared__ __align__(16) uint32_t tcme_win[];
+
+__device__ __forceinline__ uint32_t ld4s(int off)
+{
+    const uint32_t *q = tcme_win + (off >> 2);
+    return __builtin_amdgcn_alignbyte(q[1], q[0], static_cast<uint32_t>(off & 3));
 }
 
 // Inclusive sum scan inside each 16-lane DPP row; lane 16r+15 ends with the
@@ -155,6 +174,14 @@ __device__ __forceinline__ int qclass(const PicDev &R, int n, int qx, int qy)
     return 2;
 }
 
+// Part of one reference picture staged in LDS for a chunk of calls:
+// frame region [x0,x1) x [y0,y1) at `lds`, `pitch` bytes per row (x0 is a
+// multiple of 4; every row has 4 readable bytes past its end).
+struct Win {
+    int x0, y0, x1, y1, pitch;
+    bool on;
+};
+
 template <int N>
 struct Searcher {
     using Gm = Geo<N>;
@@ -164,12 +191,16 @@ struct Searcher {
     int roff[Gm::U];      // unit offset in the reference plane
     int grp;              // candidate group (N == 8)
     int o8;               // N == 8: this lane's current-block pixel
-    const uint8_t *cblk;  // current block origin
+    gptr32 cplane;        // current picture
+    size_t cblk;          // byte offset of the current block in it
     int cpitch;
+    gptr32 rplane;        // reference picture
+    Win W;                // staged reference window (W.on == false: none)
     Counts cnt;
 
-    __device__ Searcher(const PicDev &ref, const PicDev &cur, int px, int py)
-        : R(ref), cblk(cur.y + static_cast<size_t>(py) * cur.pitch + px), cpitch(cur.pitch)
+    __device__ Searcher(const PicDev &ref, const PicDev &cur, int px, int py, const Win &win)
+        : R(ref), cplane(as_global32(cur.y)), cblk(static_cast<size_t>(py) * cur.pitch + px),
+          cpitch(cur.pitch), rplane(as_global32(ref.y)), W(win)
     {
         const int lane = threadIdx.x & 63;
         grp = Gm::G == 1 ? 0 : lane / Gm::UNITS;
@@ -178,7 +209,7 @@ struct Searcher {
             const int unit = Gm::G == 1 ? lane + 64 * u : lane % Gm::UNITS;
             urow[u] = unit / Gm::DW;
             ucol[u] = (unit % Gm::DW) * 4;
-            o[u] = ld4(cur.y + static_cast<size_t>(py + urow[u]) * cur.pitch + px + ucol[u]);
+            o[u] = ld4g(cplane, cblk + static_cast<size_t>(urow[u]) * cpitch + ucol[u]);
             roff[u] = urow[u] * R.pitch + ucol[u];
         }
         o8 = 0;
@@ -186,17 +217,39 @@ struct Searcher {
             o8 = cur.y[static_cast<size_t>(py + (lane >> 3)) * cur.pitch + px + (lane & 7)];
     }
 
+    __device__ __forceinline__ bool in_win(int x, int y) const
+    {
+        return W.on && x >= W.x0 && y >= W.y0 && x + N <= W.x1 && y + N <= W.y1;
+    }
+
+    // Packed SAD of this lane's unit u for the block at (x,y): from the LDS
+    // window when the block lies inside it, else from the reference plane.
+    __device__ __forceinline__ uint32_t unit_sad(int x, int y, int u, bool lds, uint32_t acc) const
+    {
+        if (lds) {
+            const int b = (y - W.y0) * W.pitch + (x - W.x0);
+            return __builtin_amdgcn_sad_u8(ld4s(b + urow[u] * W.pitch + ucol[u]), o[u], acc);
+        }
+        const size_t b = static_cast<size_t>(y) * R.pitch + x;
+        return __builtin_amdgcn_sad_u8(ld4g(rplane, b + roff[u]), o[u], acc);
+    }
+
     // SAD of the block at full-pel (x,y), which must fit in the frame.
     __device__ __forceinline__ int32_t sad1(int x, int y) const
     {
-        const uint8_t *base = R.y + static_cast<size_t>(y) * R.pitch + x;
+        const bool lds = in_win(x, y);
         if (Gm::G == 1) {
             uint32_t acc = 0;
+            if (lds) {
 #pragma unroll
-            for (int u = 0; u < Gm::U; ++u) acc = __builtin_amdgcn_sad_u8(ld4(base + roff[u]), o[u], acc);
+                for (int u = 0; u < Gm::U; ++u) acc = unit_sad(x, y, u, true, acc);
+            } else {
+#pragma unroll
+                for (int u = 0; u < Gm::U; ++u) acc = unit_sad(x, y, u, false, acc);
+            }
             return wave_sum(static_cast<int>(acc));
         } else {
-            const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ld4(base + roff[0]), o[0], 0u)));
+            const int v = row_scan(static_cast<int>(unit_sad(x, y, 0, lds, 0u)));
             return __builtin_amdgcn_readlane(v, 15);  // group 0 priced it; all groups did
         }
     }
@@ -231,8 +284,7 @@ struct Searcher {
                     }
                 }
                 if (!live) { x = 0; y = 0; }
-                const uint8_t *base = R.y + static_cast<size_t>(y) * R.pitch + x;
-                const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ld4(base + roff[0]), o[0], 0u)));
+                const int v = row_scan(static_cast<int>(unit_sad(x, y, 0, in_win(x, y), 0u)));
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
                     const int k = s0 + g;
@@ -328,7 +380,7 @@ struct Searcher {
                 const int v = cls == 1 ? q6_px(R, x + j, y, dx, dy) : bil_px(R, x + j, y, dx, dy);
                 pk |= static_cast<uint32_t>(v) << (8 * j);
             }
-            acc = __builtin_amdgcn_sad_u8(pk, ld4(cblk + ur * cpitch + uc), acc);
+            acc = __builtin_amdgcn_sad_u8(pk, ld4g(cplane, cblk + static_cast<size_t>(ur) * cpitch + uc), acc);
         }
         return wave_sum(static_cast<int>(acc));
     }
@@ -352,9 +404,9 @@ struct Searcher {
 
 template <int N>
 __device__ void search(const PicDev &R, const PicDev &C, const tcme_call &c, tcme_result *res,
-                       tcme_stats *st)
+                       tcme_stats *st, const Win &win)
 {
-    Searcher<N> S(R, C, c.px, c.py);
+    Searcher<N> S(R, C, c.px, c.py, win);
     // Centre clamp with tc_clip's order: low bound first (motion.c:391-392).
     int bx = c.cx < 0 ? 0 : (c.cx > R.w - N ? R.w - N : c.cx);
     int by = c.cy < 0 ? 0 : (c.cy > R.h - N ? R.h - N : c.cy);
@@ -391,30 +443,29 @@ __device__ void search(const PicDev &R, const PicDev &C, const tcme_call &c, tcm
     }
 }
 
-__global__ __launch_bounds__(64) void me_search_kernel(const PicDev *__restrict__ table, int npics,
-                                                        const tcme_call *__restrict__ calls, int n,
-                                                        tcme_result *__restrict__ res,
-                                                        tcme_stats *__restrict__ stats)
+// Validates one descriptor; returns 0 and the two pictures, or a TCME_E_* code.
+__device__ __forceinline__ int check_call(const PicDev *table, int npics, const tcme_call &c,
+                                          PicDev &R, PicDev &C)
 {
-    const int i = blockIdx.x;
-    if (i >= n) return;
-    const tcme_call c = calls[i];
-    int status = 0;
+    if (c.ref_pic < 0 || c.ref_pic >= npics || c.cur_pic < 0 || c.cur_pic >= npics)
+        return TCME_E_PIC;
+    if (c.blk != 8 && c.blk != 16 && c.blk != 32 && c.blk != 64) return TCME_E_BLK;
+    R = table[c.ref_pic];
+    C = table[c.cur_pic];
+    if (!R.y || !R.hh || !C.y) return TCME_E_PIC;  // empty slot, or no half-sample planes
+    if (c.px < 0 || c.py < 0 || c.px + c.blk > C.w || c.py + c.blk > C.h) return TCME_E_ARG;
+    return 0;
+}
+
+// Runs call i with the whole wave.
+__device__ __forceinline__ void run_call(const PicDev *table, int npics, const tcme_call &c, int i,
+                                         tcme_result *res, tcme_stats *stats, const Win &win)
+{
     PicDev R{}, C{};
-    if (c.ref_pic < 0 || c.ref_pic >= npics || c.cur_pic < 0 || c.cur_pic >= npics) {
-        status = TCME_E_PIC;
-    } else if (c.blk != 8 && c.blk != 16 && c.blk != 32 && c.blk != 64) {
-        status = TCME_E_BLK;
-    } else {
-        R = table[c.ref_pic];
-        C = table[c.cur_pic];
-        if (!R.y || !R.hh || !C.y || c.px < 0 || c.py < 0 || c.px + c.blk > C.w ||
-            c.py + c.blk > C.h)
-            status = TCME_E_ARG;
-    }
+    const int status = check_call(table, npics, c, R, C);
     tcme_stats *st = stats ? stats + i : nullptr;
     if (status) {
-        if (threadIdx.x == 0) {
+        if ((threadIdx.x & 63) == 0) {
             res[i].mv_x = 0;
             res[i].mv_y = 0;
             res[i].sad = kInf;
@@ -424,10 +475,119 @@ __global__ __launch_bounds__(64) void me_search_kernel(const PicDev *__restrict_
         return;
     }
     switch (c.blk) {
-    case 8: search<8>(R, C, c, res + i, st); break;
-    case 16: search<16>(R, C, c, res + i, st); break;
-    case 32: search<32>(R, C, c, res + i, st); break;
-    default: search<64>(R, C, c, res + i, st); break;
+    case 8: search<8>(R, C, c, res + i, st, win); break;
+    case 16: search<16>(R, C, c, res + i, st, win); break;
+    case 32: search<32>(R, C, c, res + i, st, win); break;
+    default: search<64>(R, C, c, res + i, st, win); break;
+    }
+}
+
+// One wavefront per call, no staging (the single-call drop-in).
+__global__ __launch_bounds__(64) void me_search_kernel(const PicDev *__restrict__ table, int npics,
+                                                        const tcme_call *__restrict__ calls, int n,
+                                                        tcme_result *__restrict__ res,
+                                                        tcme_stats *__restrict__ stats)
+{
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const Win none{0, 0, 0, 0, 0, false};
+    run_call(table, npics, calls[i], i, res, stats, none);
+}
+
+// Block index -> chunk index so that each of the 8 XCDs (which receive
+// workgroups round-robin) works on one contiguous run of chunks: neighbouring
+// chunks read neighbouring reference rows, which then stay in that XCD's L2.
+__device__ __forceinline__ int xcd_chunk(int b, int nb)
+{
+    constexpr int kXcd = 8;
+    const int x = b % kXcd, slot = b / kXcd;
+    const int q = nb / kXcd, r = nb % kXcd;
+    return x * q + (x < r ? x : r) + slot;
+}
+
+// Relative cost of one call, for splitting a chunk between wavefronts.
+__device__ __forceinline__ int call_cost(int blk)
+{
+    return blk * blk + 256;
+}
+
+// Chunked search: a workgroup of WAVES wavefronts takes `chunk` consecutive
+// calls and stages the union of their reference windows (clamped centre +-
+// apron, one reference picture) in LDS with coalesced dword loads.  Each
+// wavefront then takes a contiguous, cost-balanced share of the chunk (calls
+// ordered largest first split evenly) and prices candidates from LDS; a
+// candidate outside the window reads the reference plane instead, so the
+// apron only affects speed, never results.  All control flow below the
+// barriers is wave-uniform: the wavefront index comes from readfirstlane and
+// the share from scalar loads of the call list.
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(
+    const PicDev *__restrict__ table, int npics, const tcme_call *__restrict__ calls, int n,
+    int chunk, int apron, int lds_cap, tcme_result *__restrict__ res,
+    tcme_stats *__restrict__ stats)
+{
+    __shared__ int box[5];
+    const int tid = threadIdx.x;
+    const int b = xcd_chunk(blockIdx.x, gridDim.x);
+    const int c0 = b * chunk;
+    const int c1 = min(n, c0 + chunk);
+    if (c0 >= n) return;  // uniform for the whole workgroup
+    if (tid == 0) {
+        box[0] = 0x7FFFFFFF; box[1] = 0x7FFFFFFF;
+        box[2] = -0x7FFFFFFF; box[3] = -0x7FFFFFFF;
+        box[4] = calls[c0].ref_pic;
+    }
+    __syncthreads();
+    const int ref0 = box[4];
+    for (int i = c0 + tid; i < c1; i += 64 * WAVES) {
+        const tcme_call c = calls[i];
+        PicDev R{}, C{};
+        if (c.ref_pic != ref0 || check_call(table, npics, c, R, C)) continue;
+        const int n_ = c.blk;
+        const int cx = c.cx < 0 ? 0 : (c.cx > R.w - n_ ? R.w - n_ : c.cx);
+        const int cy = c.cy < 0 ? 0 : (c.cy > R.h - n_ ? R.h - n_ : c.cy);
+        atomicMin(&box[0], cx - apron);
+        atomicMin(&box[1], cy - apron);
+        atomicMax(&box[2], cx + n_ + apron);
+        atomicMax(&box[3], cy + n_ + apron);
+    }
+    __syncthreads();
+    Win win{0, 0, 0, 0, 0, false};
+    if (box[0] != 0x7FFFFFFF) {
+        const PicDev R = table[ref0];
+        const int x0 = (box[0] < 0 ? 0 : box[0]) & ~3;
+        const int y0 = box[1] < 0 ? 0 : box[1];
+        const int x1 = box[2] > R.w ? R.w : box[2];
+        const int y1 = box[3] > R.h ? R.h : box[3];
+        const int pitch = ((x1 - x0 + 3) & ~3) + 4;
+        const int rows = y1 - y0;
+        if (x1 > x0 && rows > 0 && rows * pitch + 8 <= lds_cap) {
+            const int words = pitch / 4;
+            gptr32 src = as_global32(R.y);
+            for (int i = tid; i < rows * words; i += 64 * WAVES) {
+                const int r = i / words, cw = i - r * words;
+                tcme_win[i] = src[(static_cast<size_t>(y0 + r) * R.pitch + x0) / 4 + cw];
+            }
+            if (tid == 0) tcme_win[rows * words] = 0;  // the pair read past the last row
+            win = Win{x0, y0, x1, y1, pitch, true};
+        }
+    }
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int total = 0;
+    for (int j = c0; j < c1; ++j) total += call_cost(calls[j].blk);
+    const int lo = static_cast<int>(static_cast<long long>(total) * wave / WAVES);
+    const int hi = static_cast<int>(static_cast<long long>(total) * (wave + 1) / WAVES);
+    int acc = 0;
+    for (int j = c0; j < c1; ++j) {
+        const tcme_call c = calls[j];
+        const int cj = call_cost(c.blk);
+        const int mid = acc + cj / 2;
+        acc += cj;
+        if (mid < lo || mid >= hi) continue;
+        Win w = win;
+        w.on = win.on && c.ref_pic == ref0;
+        run_call(table, npics, c, j, res, stats, w);
     }
 }
 
@@ -482,6 +642,35 @@ hipError_t launch_search(const PicDev *table, int npics, const tcme_call *calls,
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(me_search_kernel, dim3(n), dim3(64), 0, s, table, npics, calls, n, res,
                        stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_search_tiled(const PicDev *table, int npics, const tcme_call *calls, int n,
+                               const Tiling &t, tcme_result *res, tcme_stats *stats,
+                               hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (t.chunk == 1 && t.lds_bytes == 0)  // the plain form: one wavefront per call
+        return launch_search(table, npics, calls, n, res, stats, s);
+    const int nb = (n + t.chunk - 1) / t.chunk;
+    switch (t.waves) {
+    case 1:
+        hipLaunchKernelGGL(me_tile_kernel<1>, dim3(nb), dim3(64), t.lds_bytes, s, table, npics,
+                           calls, n, t.chunk, t.apron, t.lds_bytes, res, stats);
+        break;
+    case 2:
+        hipLaunchKernelGGL(me_tile_kernel<2>, dim3(nb), dim3(128), t.lds_bytes, s, table, npics,
+                           calls, n, t.chunk, t.apron, t.lds_bytes, res, stats);
+        break;
+    case 8:
+        hipLaunchKernelGGL(me_tile_kernel<8>, dim3(nb), dim3(512), t.lds_bytes, s, table, npics,
+                           calls, n, t.chunk, t.apron, t.lds_bytes, res, stats);
+        break;
+    default:
+        hipLaunchKernelGGL(me_tile_kernel<4>, dim3(nb), dim3(256), t.lds_bytes, s, table, npics,
+                           calls, n, t.chunk, t.apron, t.lds_bytes, res, stats);
+        break;
+    }
     return hipGetLastError();
 }
 

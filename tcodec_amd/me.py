@@ -27,6 +27,7 @@ EXPORTS = (
     "tc_motion_est", "tcme_last_error", "tcme_version", "tcme_create", "tcme_destroy",
     "tcme_pic_create", "tcme_pic_destroy", "tcme_pic_upload", "tcme_pic_upload_device",
     "tcme_pic_luma", "tcme_pic_finish", "tcme_search", "tcme_search_host",
+    "tcme_set_tiling", "tcme_get_tiling", "tcme_autotune",
 )
 
 CALL_DTYPE = np.dtype([("ref_pic", "<i4"), ("cur_pic", "<i4"), ("px", "<i4"), ("py", "<i4"),
@@ -41,6 +42,14 @@ i32p = C.POINTER(C.c_int32)
 
 class MV(C.Structure):
     _fields_ = [("x", C.c_int32), ("y", C.c_int32)]
+
+
+class Tiling(C.Structure):
+    _fields_ = [("chunk", C.c_int32), ("waves", C.c_int32), ("apron", C.c_int32),
+                ("lds_bytes", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class TcmeError(RuntimeError):
@@ -82,6 +91,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
                               C.c_void_p]
     L.tcme_search_host.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_void_p]
+    L.tcme_set_tiling.argtypes = [C.c_void_p, C.POINTER(Tiling)]
+    L.tcme_get_tiling.argtypes = [C.c_void_p, C.POINTER(Tiling)]
+    L.tcme_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                C.c_void_p, C.POINTER(Tiling), C.POINTER(C.c_float)]
     _lib = L
     return L
 
@@ -105,9 +118,10 @@ def motion_est(ref: np.ndarray, ref_stride: int, ref_w: int, ref_h: int, orig: n
     ref = np.ascontiguousarray(ref, np.uint8)
     orig = np.ascontiguousarray(orig, np.uint8)
     sad = C.c_int32()
-    mv = L.tc_motion_est(ref.ctypes.data_as(u8p), ref_stride, ref_w, ref_h,
-                         C.cast(orig.ctypes.data + orig_off, u8p), orig_stride, center_x,
-                         center_y, blk_size, search_range, C.byref(sad))
+    mv = L.tc_motion_est(ref.ctypes.data_as(u8p), int(ref_stride), int(ref_w), int(ref_h),
+                         C.cast(orig.ctypes.data + int(orig_off), u8p), int(orig_stride),
+                         int(center_x), int(center_y), int(blk_size), int(search_range),
+                         C.byref(sad))
     return (mv.x, mv.y), sad.value
 
 
@@ -170,6 +184,24 @@ class Engine:
         _check(self.L.tcme_search_host(self.h, calls.ctypes.data, n, res.ctypes.data,
                                        st.ctypes.data if stats else None, C.c_void_p(stream)))
         return (res, st) if stats else res
+
+    def tiling(self) -> Tiling:
+        t = Tiling()
+        _check(self.L.tcme_get_tiling(self.h, C.byref(t)))
+        return t
+
+    def set_tiling(self, chunk: int, waves: int, apron: int, lds_bytes: int) -> None:
+        t = Tiling(chunk, waves, apron, lds_bytes)
+        _check(self.L.tcme_set_tiling(self.h, C.byref(t)))
+
+    def autotune(self, calls_ptr: int, n: int, res_ptr: int, reps: int = 3, stream: int = 0):
+        """Picks the fastest launch shape for this device call list; returns
+        (Tiling, ms per launch)."""
+        t = Tiling()
+        ms = C.c_float()
+        _check(self.L.tcme_autotune(self.h, C.c_void_p(calls_ptr), n, C.c_void_p(res_ptr), reps,
+                                    C.c_void_p(stream), C.byref(t), C.byref(ms)))
+        return t, ms.value
 
     def search_device(self, calls_ptr: int, n: int, res_ptr: int, stats_ptr: int | None = None,
                       stream: int = 0) -> None:

@@ -95,3 +95,31 @@ def algo_bytes(stats: np.ndarray, blk: np.ndarray) -> int:
     return int((stats["int_evals"].astype(np.int64) * 2 * nn +
                 stats["q6_evals"].astype(np.int64) * (nn + (n + 5) ** 2) +
                 stats["qbil_evals"].astype(np.int64) * (nn + (n + 1) ** 2)).sum())
+
+
+def frame_calls_by_ctu(w: int, h: int, ref_pics, cur_pic: int, sr: int,
+                       sizes=SIZES) -> np.ndarray:
+    """The same call set as ``frame_calls(qt_nodes(w, h), ...)`` ordered for
+    the chunked kernel: grouped by (CTU, reference), complete CTUs (85 nodes)
+    first and partial edge CTUs last, largest blocks first inside a group, so
+    a chunk of 85 calls is exactly one CTU searched against one reference."""
+    nodes = qt_nodes(w, h, sizes)
+    ctu = (nodes[:, 1] // 64) * ((w + 63) // 64) + nodes[:, 0] // 64
+    counts = np.bincount(ctu, minlength=int(ctu.max()) + 1)
+    partial = (counts[ctu] != 85).astype(np.int64)
+    refs = np.asarray(list(ref_pics), np.int32)
+    m = nodes.shape[0]
+    r_idx = np.repeat(np.arange(refs.size), m)
+    nd = np.tile(np.arange(m), refs.size)
+    order = np.lexsort((nodes[nd, 0], nodes[nd, 1], -nodes[nd, 2], r_idx, ctu[nd], partial[nd]))
+    out = np.zeros(m * refs.size, CALL_DTYPE)
+    sel = nd[order]
+    out["ref_pic"] = refs[r_idx[order]]
+    out["cur_pic"] = cur_pic
+    out["px"] = nodes[sel, 0]
+    out["py"] = nodes[sel, 1]
+    out["cx"] = nodes[sel, 0]
+    out["cy"] = nodes[sel, 1]
+    out["blk"] = nodes[sel, 2]
+    out["sr"] = sr
+    return out

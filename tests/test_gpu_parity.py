@@ -176,3 +176,66 @@ def test_dropin_symbol_matches_oracle(orc):
         new = np.ascontiguousarray(rbuf[:, :w])
         (mx, my), sad = me.motion_est(rbuf, stride, w, h, cur, 0, w, 40, 40, 16, 24)
         assert (mx, my, sad) == orc.motion_est(new, cur, 0, 0, 40, 40, 16, 24)
+
+
+@pytest.mark.parametrize("tiling", [(1, 1, 0, 0), (85, 4, 40, 48 * 1024), (43, 2, 24, 24 * 1024),
+                                    (170, 8, 56, 64 * 1024), (85, 4, 200, 4096),
+                                    (7, 2, 8, 16 * 1024)],
+                         ids=lambda t: "c%d_w%d_a%d_l%d" % t)
+def test_gpu_tilings_agree_with_oracle(engine, orc, tiling):
+    """Every launch shape gives the oracle's answers: LDS staging on and off,
+    windows that overflow the LDS budget, chunks that straddle CTUs and
+    references (CTU-major and reference-major orders)."""
+    from tcodec_amd import synth, workload
+    log(f"tiling {tiling}")
+    fr = synth.scene_video(256, 200, 5, seed=9, rects=6, pan=(0.75, 0.25))
+    refs = [fr[i][0] for i in (0, 1, 3)]
+    cur = fr[4][0]
+    rps = [engine.pic(r) for r in refs]
+    cp = engine.pic(cur)
+    old = engine.tiling()
+    engine.set_tiling(*tiling)
+    try:
+        for order in ("ctu", "ref"):
+            if order == "ctu":
+                calls = workload.frame_calls_by_ctu(256, 200, rps, cp, workload.SR_B)
+            else:
+                calls = workload.frame_calls(workload.qt_nodes(256, 200), rps, cp, workload.SR_P)
+            res, st = engine.search(calls, stats=True)
+            for k, rp in enumerate(rps):
+                m = calls["ref_pic"] == rp
+                want, wst = oracle_batch(orc, refs[k], cur, calls[m], stats=True)
+                compare(res[m], want, calls[m], f"{tiling} {order} ref {k}")
+                assert np.array_equal(st["int_evals"][m], wst[:, 0])
+                assert np.array_equal(st["q6_evals"][m], wst[:, 2])
+    finally:
+        engine.set_tiling(old.chunk, old.waves, old.apron, old.lds_bytes)
+        for p in rps + [cp]:
+            engine.free(p)
+
+
+def test_gpu_autotune_keeps_results(engine, orc):
+    """The autotuner only changes speed: after it runs, results are unchanged."""
+    import torch
+    from tcodec_amd import me, synth, workload
+    log("autotune")
+    fr = synth.scene_video(320, 240, 3, seed=2, rects=6)
+    ref, cur = fr[0][0], fr[2][0]
+    rp, cp = engine.pic(ref), engine.pic(cur)
+    calls = workload.frame_calls_by_ctu(320, 240, [rp], cp, workload.SR_B)
+    d_calls = torch.from_numpy(calls.view(np.int32).reshape(-1, 8).copy()).cuda()
+    d_res = torch.zeros((len(calls), 4), dtype=torch.int32, device="cuda")
+    old = engine.tiling()
+    try:
+        best, ms = engine.autotune(d_calls.data_ptr(), len(calls), d_res.data_ptr(), reps=1,
+                                   stream=torch.cuda.current_stream().cuda_stream)
+        assert ms > 0 and best.waves in (1, 2, 4, 8)
+        assert engine.tiling().as_dict() == best.as_dict()
+        torch.cuda.synchronize()
+        got = d_res.cpu().numpy().view(me.RESULT_DTYPE).ravel()
+        compare(got, oracle_batch(orc, ref, cur, calls), calls, "autotune")
+        compare(engine.search(calls), oracle_batch(orc, ref, cur, calls), calls, "after tune")
+    finally:
+        engine.set_tiling(old.chunk, old.waves, old.apron, old.lds_bytes)
+        engine.free(rp)
+        engine.free(cp)
