@@ -96,6 +96,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tiling", default=None,
+                    help="chunk,waves,apron,lds_bytes: use this launch shape instead of autotuning")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (counter calibration)")
     ap.add_argument("--mode", choices=("tuned", "plain"), default="tuned",
                     help="plain: one wavefront per call only; tuned: also the autotuned "
                          "chunked launch (reported as value)")
@@ -112,6 +116,13 @@ def main():
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_mib} MiB device to device")
+        del src, dst
     eng = me.Engine(0)
     log("generating and uploading 13 frames of 1920x1080")
     scene, pics, frames = build_workload(eng)
@@ -163,7 +174,11 @@ def main():
     # plain form: one wavefront per call, no LDS staging
     eng.set_tiling(1, 1, 0, 0)
     plain_el, plain_lat = timed("plain")
-    if args.mode == "tuned":
+    if args.tiling:
+        eng.set_tiling(*[int(v) for v in args.tiling.split(",")])
+        best = eng.tiling()
+        elapsed, lat = timed("fixed")
+    elif args.mode == "tuned":
         # tuned form: the autotuner picks the launch shape on frame 0
         best, best_ms = eng.autotune(d_calls[0].data_ptr(), len(frames[0][2]),
                                      d_res[0].data_ptr(), reps=2, stream=sptr)

@@ -65,6 +65,20 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", os.path.join(HERE, "csrc")], check=True)
 
 
+def _share_torch_runtime() -> None:
+    """If PyTorch is importable, initialise its HIP runtime before libtcme.so is
+    loaded.  torch ships its own libamdhip64 with the same soname, so loading
+    torch first makes both share one runtime (device pointers and streams from
+    torch are then valid for the library); loading libtcme.so first would give
+    the process two HIP runtimes and torch would then find no GPU."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def load(path: str = LIB_PATH) -> C.CDLL:
     """Load the library (once).  Raises if it has not been built."""
     global _lib
@@ -72,6 +86,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         return _lib
     if not os.path.exists(path):
         raise TcmeError(f"{path} is not built: run __graft_entry__.build()")
+    _share_torch_runtime()
     L = C.CDLL(path)
     L.tc_motion_est.restype = MV
     L.tc_motion_est.argtypes = [u8p, C.c_int, C.c_int, C.c_int, u8p, C.c_int, C.c_int, C.c_int,
