@@ -51,6 +51,24 @@ def build_workload(eng):
     return scene, pics, frames
 
 
+def kernel_name(tiling: dict) -> str:
+    plain = tiling["chunk"] == 1 and tiling["lds_bytes"] == 0
+    return "me_search_kernel" if plain else "me_tile_kernel"
+
+
+def measured_traffic(tiling: dict):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
+    committed PMC summary, when it was taken with this launch shape."""
+    path = os.path.join(ROOT, "profiles", "me_search_traffic.json")
+    try:
+        t = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if t.get("tiling") != tiling:
+        return None
+    return int(t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"])
+
+
 def cpu_baseline(scene, pics, frames, budget_s=12.0):
     """The reference's own tc_motion_est (compiled from its sources into
     oracle/_ref/libtcref.so), one thread, on a seeded sample of one frame's
@@ -220,9 +238,10 @@ def main():
         "tiling": best.as_dict(),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": measured_traffic(best.as_dict()),
                      "algo_bytes_per_frame": int(np.mean(algo)),
-                     "kernel": "me_search_kernel", "kernel_ms_avg": round(kernel_s * 1e3, 3)},
+                     "kernel": kernel_name(best.as_dict()),
+                     "kernel_ms_avg": round(kernel_s * 1e3, 3)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
