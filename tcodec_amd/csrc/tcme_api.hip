@@ -51,9 +51,9 @@ int fail(int code, const char *fmt, ...)
 
 struct Pic {
     bool used = false;
-    int w = 0, h = 0, pitch = 0, hpitch = 0;
+    int w = 0, h = 0, pitch = 0, apitch = 0;
     uint8_t *d_y = nullptr;
-    int16_t *d_half = nullptr;  // hh | vv | cc, each hpitch*h
+    uint64_t *d_an = nullptr;  // anchors, apitch * h entries
 };
 
 int align_up(int v, int a) { return (v + a - 1) / a * a; }
@@ -71,7 +71,13 @@ struct tcme_ctx {
     tcme_stats *d_stats = nullptr;
     size_t stage_cap = 0;
     tcme::Tiling tiling;
+    // plain form: the four per-size kernels run concurrently on these streams,
+    // forked from and joined back into the caller's stream with events
+    hipStream_t sub[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
     std::mutex mu;
+    std::mutex launch_mu;
 };
 
 namespace {
@@ -87,13 +93,11 @@ tcme::PicDev dev_view(const Pic &p)
     tcme::PicDev d{};
     if (!p.used) return d;
     d.y = p.d_y;
-    d.hh = p.d_half;
-    d.vv = p.d_half ? p.d_half + static_cast<size_t>(p.hpitch) * p.h : nullptr;
-    d.cc = p.d_half ? p.d_half + 2 * static_cast<size_t>(p.hpitch) * p.h : nullptr;
+    d.an = p.d_an;
     d.w = p.w;
     d.h = p.h;
     d.pitch = p.pitch;
-    d.hpitch = p.hpitch;
+    d.apitch = p.apitch;
     return d;
 }
 
@@ -117,18 +121,18 @@ int pic_alloc(tcme_ctx *c, int w, int h, bool half)
     p.w = w;
     p.h = h;
     p.pitch = align_up(w, 64);
-    p.hpitch = align_up(w, 64);
+    p.apitch = align_up(w, 64);
     size_t ybytes = static_cast<size_t>(p.pitch) * h + tcme::kTailSlack;
     if (hipMalloc(&p.d_y, ybytes) != hipSuccess)
         return fail(TCME_E_NOMEM, "hipMalloc luma %zu bytes", ybytes);
     hipMemset(p.d_y, 0, ybytes);
     if (half) {
-        size_t hbytes = 3 * static_cast<size_t>(p.hpitch) * h * sizeof(int16_t) + tcme::kTailSlack;
-        if (hipMalloc(&p.d_half, hbytes) != hipSuccess) {
+        size_t abytes = static_cast<size_t>(p.apitch) * h * sizeof(uint64_t) + tcme::kTailSlack;
+        if (hipMalloc(&p.d_an, abytes) != hipSuccess) {
             hipFree(p.d_y);
-            return fail(TCME_E_NOMEM, "hipMalloc half-sample planes %zu bytes", hbytes);
+            return fail(TCME_E_NOMEM, "hipMalloc anchor plane %zu bytes", abytes);
         }
-        hipMemset(p.d_half, 0, hbytes);
+        hipMemset(p.d_an, 0, abytes);
     }
     p.used = true;
     c->pics[id] = p;
@@ -141,18 +145,40 @@ int pic_free(tcme_ctx *c, int id)
 {
     Pic &p = c->pics[id];
     hipFree(p.d_y);
-    if (p.d_half) hipFree(p.d_half);
+    if (p.d_an) hipFree(p.d_an);
     p = Pic();
     tcme::PicDev d{};
     HIP_TRY(hipMemcpy(c->d_table + id, &d, sizeof d, hipMemcpyHostToDevice));
     return TCME_OK;
 }
 
+// Launches the search of n calls in launch shape t on stream s.
+int dispatch(tcme_ctx *c, const tcme_call *calls, int n, const tcme::Tiling &t,
+             tcme_result *res, tcme_stats *stats, hipStream_t s)
+{
+    if (n <= 0) return TCME_OK;
+    if (t.chunk > 1 || t.lds_bytes > 0) {
+        HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, t, res, stats, s));
+        return TCME_OK;
+    }
+    std::lock_guard<std::mutex> g(c->launch_mu);
+    static const int sizes[4] = {64, 32, 16, 8};
+    HIP_TRY(hipEventRecord(c->fork, s));
+    for (int k = 0; k < 4; ++k) {
+        HIP_TRY(hipStreamWaitEvent(c->sub[k], c->fork, 0));
+        HIP_TRY(tcme::launch_search_n(sizes[k], c->d_table, c->max_pics, calls, n, res, stats,
+                                      c->sub[k]));
+        HIP_TRY(hipEventRecord(c->join[k], c->sub[k]));
+        HIP_TRY(hipStreamWaitEvent(s, c->join[k], 0));
+    }
+    return TCME_OK;
+}
+
 int pic_derive(tcme_ctx *c, int id, hipStream_t s)
 {
     const Pic &p = c->pics[id];
-    if (!p.d_half) return TCME_OK;
-    HIP_TRY(tcme::launch_halfpel(dev_view(p), s));
+    if (!p.d_an) return TCME_OK;
+    HIP_TRY(tcme::launch_anchors(dev_view(p), s));
     return TCME_OK;
 }
 
@@ -182,6 +208,14 @@ TCME_API int tcme_create(tcme_ctx **out, int device, int max_pics)
         return fail(TCME_E_NOMEM, "picture table");
     }
     hipMemset(c->d_table, 0, sizeof(tcme::PicDev) * max_pics);
+    bool ok = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < 4 && ok; ++k)
+        ok = hipStreamCreateWithFlags(&c->sub[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        tcme_destroy(c);
+        return fail(TCME_E_HIP, "context streams/events");
+    }
     *out = c;
     return TCME_OK;
 }
@@ -197,6 +231,11 @@ TCME_API void tcme_destroy(tcme_ctx *c)
     hipFree(c->d_calls);
     hipFree(c->d_res);
     hipFree(c->d_stats);
+    for (int k = 0; k < 4; ++k) {
+        if (c->sub[k]) hipStreamDestroy(c->sub[k]);
+        if (c->join[k]) hipEventDestroy(c->join[k]);
+    }
+    if (c->fork) hipEventDestroy(c->fork);
     delete c;
 }
 
@@ -263,9 +302,7 @@ TCME_API int tcme_search(tcme_ctx *c, const tcme_call *calls, int n, tcme_result
     if (n < 0 || (n > 0 && (!calls || !results))) return fail(TCME_E_ARG, "bad call list");
     if (n == 0) return TCME_OK;
     if (set_device(c) != TCME_OK) return TCME_E_HIP;
-    HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, c->tiling, results,
-                                      stats, static_cast<hipStream_t>(stream)));
-    return TCME_OK;
+    return dispatch(c, calls, n, c->tiling, results, stats, static_cast<hipStream_t>(stream));
 }
 
 namespace {
@@ -333,12 +370,10 @@ TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_resu
     float best_t = 1e30f;
     {
         const tcme::Tiling plain;
-        HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, plain, results,
-                                          nullptr, s));
+        if (int e = dispatch(c, calls, n, plain, results, nullptr, s)) return e;
         HIP_TRY(hipEventRecord(e0, s));
         for (int r = 0; r < reps; ++r)
-            HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, plain, results,
-                                              nullptr, s));
+            if (int e = dispatch(c, calls, n, plain, results, nullptr, s)) return e;
         HIP_TRY(hipEventRecord(e1, s));
         HIP_TRY(hipEventSynchronize(e1));
         HIP_TRY(hipEventElapsedTime(&best_t, e0, e1));
@@ -355,12 +390,10 @@ TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_resu
                 int lds = (span + 2 * ap + 12) * (64 + 2 * ap) + 64;
                 lds = (lds + 1023) / 1024 * 1024;
                 t.lds_bytes = lds > 64 * 1024 ? 64 * 1024 : lds;
-                HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, t, results,
-                                                  nullptr, s));  // warm
+                if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;  // warm
                 HIP_TRY(hipEventRecord(e0, s));
                 for (int r = 0; r < reps; ++r)
-                    HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, t,
-                                                      results, nullptr, s));
+                    if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;
                 HIP_TRY(hipEventRecord(e1, s));
                 HIP_TRY(hipEventSynchronize(e1));
                 float ms = 0;
@@ -399,8 +432,8 @@ TCME_API int tcme_search_host(tcme_ctx *c, const tcme_call *calls, int n, tcme_r
         c->stage_cap = cap;
     }
     HIP_TRY(hipMemcpyAsync(c->d_calls, calls, n * sizeof(tcme_call), hipMemcpyHostToDevice, s));
-    HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, c->d_calls, n, c->tiling,
-                                      c->d_res, stats ? c->d_stats : nullptr, s));
+    if (int e = dispatch(c, c->d_calls, n, c->tiling, c->d_res, stats ? c->d_stats : nullptr, s))
+        return e;
     HIP_TRY(hipMemcpyAsync(results, c->d_res, n * sizeof(tcme_result), hipMemcpyDeviceToHost, s));
     if (stats)
         HIP_TRY(hipMemcpyAsync(stats, c->d_stats, n * sizeof(tcme_stats), hipMemcpyDeviceToHost, s));
@@ -549,7 +582,7 @@ TCME_API tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, in
     if (hipMemcpyAsync(d.d_call, d.h_stage, sizeof call, hipMemcpyHostToDevice, d.stream) != hipSuccess ||
         hipMemcpy2DAsync(d.d_blk, d.blk_pitch, blk, n, n, n, hipMemcpyHostToDevice, d.stream) != hipSuccess)
         die("copy block");
-    if (tcme::launch_search(d.ctx->d_table, d.ctx->max_pics, d.d_call, 1, d.d_res, nullptr, d.stream) != hipSuccess)
+    if (tcme::launch_search_n(n, d.ctx->d_table, d.ctx->max_pics, d.d_call, 1, d.d_res, nullptr, d.stream) != hipSuccess)
         die("launch");
     if (hipMemcpyAsync(d.h_res, d.d_res, sizeof(tcme_result), hipMemcpyDeviceToHost, d.stream) != hipSuccess ||
         hipStreamSynchronize(d.stream) != hipSuccess)

@@ -10,14 +10,15 @@ namespace tcme {
 // Device view of one picture.  The luma plane has `pitch` bytes per row and at
 // least kTailSlack readable bytes after the last row, so a 4-byte read at any
 // in-frame column may fetch the whole aligned 8-byte pair that covers it.
-// hh/vv/cc are the raw (unclipped) six-tap half-sample planes, int16, `hpitch`
-// elements per row: hh(x,y) between (x,y) and (x+1,y); vv(x,y) between (x,y)
-// and (x,y+1); cc(x,y) the centre of the four.  Entries whose taps would leave
-// the frame hold 0 and are never read by the search or the predictor.
+// `an` (reference pictures only) holds one 8-byte anchor per sample, `apitch`
+// entries per row: bits 0-15 sample, 16-31 hh, 32-47 vv, 48-63 cc, with hh, vv, cc
+// the raw (unclipped) six-tap half samples right of, below and diagonal to the
+// sample.  Half samples whose taps would leave the frame hold 0 and are never
+// read by the search or the predictor.
 struct PicDev {
     const uint8_t *y;
-    const int16_t *hh, *vv, *cc;
-    int32_t w, h, pitch, hpitch;
+    const uint64_t *an;
+    int32_t w, h, pitch, apitch;
 };
 
 constexpr int kTailSlack = 256;
@@ -34,9 +35,12 @@ struct Tiling {
     int lds_bytes = 0;
 };
 
-hipError_t launch_halfpel(const PicDev &p, hipStream_t s);
-hipError_t launch_search(const PicDev *table, int npics, const tcme_call *calls, int n,
-                         tcme_result *res, tcme_stats *stats, hipStream_t s);
+hipError_t launch_anchors(const PicDev &p, hipStream_t s);
+// One plain-form kernel (one wavefront per call) for the calls of block size
+// blk in the list; launch all four sizes to cover a mixed list.
+hipError_t launch_search_n(int blk, const PicDev *table, int npics, const tcme_call *calls, int n,
+                           tcme_result *res, tcme_stats *stats, hipStream_t s);
+// The chunked kernel (Tiling::chunk > 1 or LDS in use).
 hipError_t launch_search_tiled(const PicDev *table, int npics, const tcme_call *calls, int n,
                                const Tiling &t, tcme_result *res, tcme_stats *stats,
                                hipStream_t s);

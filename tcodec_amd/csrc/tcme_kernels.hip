@@ -1,18 +1,25 @@
 // gfx950 kernels for TCodec's motion-estimation path.
 //
-//   halfpel_kernel    raw six-tap half-sample planes of a reference picture
+//   anchor_kernel     per reference picture: every sample with its raw six-tap
+//                     half samples, packed 8 bytes per sample (reference
+//                     src/motion.c:70-107)
 //   me_search_kernel  the hierarchical hexagon search + quarter-pel refinement,
 //                     one wavefront per call (reference src/motion.c:384-485)
+//   me_tile_kernel    the same search for a chunk of calls per workgroup, with
+//                     the union of their reference windows staged in LDS
 //
-// The search is the reference's sequential greedy walk, made wave-parallel
-// without changing its answer: every candidate of a pattern pass is priced at
-// once around the current best, the FIRST candidate (table order) that is
-// strictly better is taken, and the candidates after it are priced again
-// around the new best.  That is exactly what the sequential loop computes,
-// because the candidates before the first improvement saw the same centre.
+// The walk is the reference's sequential greedy walk.  Its decisions are
+// wave-uniform (scalar unit); lanes price candidates.  Candidates may be
+// priced speculatively in batches around the current best: the FIRST one in
+// table order that is strictly better is taken and those after it are priced
+// again around the new best.  That is exactly what the sequential loop
+// computes, because the candidates before the first improvement saw the same
+// centre, so results are bit-identical whatever the batch size.
 //
-// Block costs use v_sad_u8 on packed bytes (4 |a-b| per lane-op) and DPP
-// row reductions; the walk's control flow is wave-uniform (scalar unit).
+// Integer block costs: v_sad_u8 on packed bytes (4 |a-b| per lane-op) and a
+// DPP row scan.  Quarter-pel: every candidate of one call lies within one
+// sample of the integer best (fx in {X-1, X}, fy in {Y-1, Y}), so a lane loads
+// its 3-row anchor neighbourhood once and prices candidates from registers.
 #include "tcme_internal.h"
 
 namespace tcme {
@@ -25,14 +32,16 @@ __device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f)
     return (a + f - 5 * (b + e) + 20 * (c + d) + 16) >> 5;
 }
 
-// Global-memory view of a plane (address space 1), so plane reads compile to
-// global loads rather than flat ones.
+// Global-memory views (address space 1), so plane reads compile to global
+// loads rather than flat ones.
 typedef const __attribute__((address_space(1))) uint32_t *gptr32;
+typedef const __attribute__((address_space(1))) uint64_t *gptr64;
 
-__device__ __forceinline__ gptr32 as_global32(const void *p)
-{
-    return (gptr32)(p);
-}
+__device__ __forceinline__ gptr32 as_global32(const void *p) { return (gptr32)(p); }
+__device__ __forceinline__ gptr64 as_global64(const void *p) { return (gptr64)(p); }
+
+// One anchor entry: bits 0-15 sample, 16-31 hh, 32-47 vv, 48-63 cc.
+typedef uint64_t anc_t;
 
 // Four bytes at byte offset `off` of a 4-byte-aligned buffer, assembled from
 // the aligned dword pair that covers them.  Offsets, never integer-cast
@@ -95,13 +104,15 @@ template <> struct Pat<PAT_QPEL> {
 };
 
 // Lane geometry for an N x N block held as dword units (4 pixels each).
-// N >= 16: every lane owns U units of one candidate (G == 1).
+// N >= 16: every lane owns U units of one candidate (G == 1); integer
+//          candidates are priced B at a time.
 // N == 8 : 16 units, so the wave's four DPP rows price four candidates (G == 4).
 template <int N> struct Geo {
     static constexpr int DW = N / 4;
     static constexpr int UNITS = N * DW;
     static constexpr int U = UNITS >= 64 ? UNITS / 64 : 1;
     static constexpr int G = UNITS >= 64 ? 1 : 64 / UNITS;
+    static constexpr int B = N >= 64 ? 2 : 6;
 };
 
 struct Counts { int ie = 0, oob = 0, q6 = 0, qb = 0; };
@@ -111,46 +122,32 @@ __device__ __forceinline__ bool fits(const PicDev &R, int n, int x, int y)
     return x >= 0 && y >= 0 && x + n <= R.w && y + n <= R.h;
 }
 
-// One six-tap quarter-pel sample from the nine anchors, single final clip
-// (reference src/motion.c:193-260).  dx, dy are wave-uniform.
-__device__ __forceinline__ int q6_px(const PicDev &R, int x, int y, int dx, int dy)
+// ---- anchors ---------------------------------------------------------------
+// Entry (x,y): sample i(x,y), then hh/vv/cc, the raw six-tap half samples
+// right of, below and diagonal to the sample (all fit int16).
+__device__ __forceinline__ int a_i(anc_t e) { return static_cast<int>(static_cast<uint32_t>(e) & 0xFFFFu); }
+__device__ __forceinline__ int a_h(anc_t e) { return static_cast<int>(static_cast<uint32_t>(e)) >> 16; }
+__device__ __forceinline__ int a_v(anc_t e) { return static_cast<int>(static_cast<uint32_t>(e >> 32) << 16) >> 16; }
+__device__ __forceinline__ int a_c(anc_t e) { return static_cast<int>(static_cast<uint32_t>(e >> 32)) >> 16; }
+
+// Quarter-sample value from the four anchor entries around it, raw
+// intermediates and a single final clip (reference src/motion.c:193-260).
+// Each anchor row is a lattice L0 (sample), L1 (half), L2 (next sample); a
+// phase p picks L0, avg(L0,L1), L1, avg(L1,L2); with a == b for even phases,
+// (a + b + 1) >> 1 == a, so one expression covers all four.
+__device__ __forceinline__ int lattice(int p, int l0, int l1, int l2)
 {
-    const uint8_t *yp = R.y + static_cast<size_t>(y) * R.pitch + x;
-    const size_t ho = static_cast<size_t>(y) * R.hpitch + x;
-    int top = 0, mid = 0, bot = 0;
-    if (dy <= 1) {  // integer row y: i00, hh, i10
-        switch (dx) {
-        case 0: top = yp[0]; break;
-        case 1: top = (yp[0] + R.hh[ho] + 1) >> 1; break;
-        case 2: top = R.hh[ho]; break;
-        default: top = (R.hh[ho] + yp[1] + 1) >> 1; break;
-        }
-    }
-    if (dy >= 1) {  // half row: vv(x), cc, vv(x+1)
-        switch (dx) {
-        case 0: mid = R.vv[ho]; break;
-        case 1: mid = (R.vv[ho] + R.cc[ho] + 1) >> 1; break;
-        case 2: mid = R.cc[ho]; break;
-        default: mid = (R.cc[ho] + R.vv[ho + 1] + 1) >> 1; break;
-        }
-    }
-    if (dy == 3) {  // integer row y+1: i01, hh(y+1), i11
-        const uint8_t *yq = yp + R.pitch;
-        const size_t h1 = ho + R.hpitch;
-        switch (dx) {
-        case 0: bot = yq[0]; break;
-        case 1: bot = (yq[0] + R.hh[h1] + 1) >> 1; break;
-        case 2: bot = R.hh[h1]; break;
-        default: bot = (R.hh[h1] + yq[1] + 1) >> 1; break;
-        }
-    }
-    int v;
-    switch (dy) {
-    case 0: v = top; break;
-    case 1: v = (top + mid + 1) >> 1; break;
-    case 2: v = mid; break;
-    default: v = (mid + bot + 1) >> 1; break;
-    }
+    const int a = p >= 2 ? l1 : l0;
+    const int b = p == 1 ? l1 : (p == 3 ? l2 : a);
+    return (a + b + 1) >> 1;
+}
+
+__device__ __forceinline__ int qpix(anc_t p00, anc_t p10, anc_t p01, anc_t p11, int dx, int dy)
+{
+    const int top = lattice(dx, a_i(p00), a_h(p00), a_i(p10));
+    const int mid = lattice(dx, a_v(p00), a_c(p00), a_v(p10));
+    const int bot = lattice(dx, a_i(p01), a_h(p01), a_i(p11));
+    const int v = lattice(dy, top, mid, bot);
     return v < 0 ? 0 : (v > 255 ? 255 : v);
 }
 
@@ -175,8 +172,8 @@ __device__ __forceinline__ int qclass(const PicDev &R, int n, int qx, int qy)
 }
 
 // Part of one reference picture staged in LDS for a chunk of calls:
-// frame region [x0,x1) x [y0,y1) at `lds`, `pitch` bytes per row (x0 is a
-// multiple of 4; every row has 4 readable bytes past its end).
+// frame region [x0,x1) x [y0,y1), `pitch` bytes per row (x0 is a multiple of
+// 4; every row has 4 readable bytes past its end).
 struct Win {
     int x0, y0, x1, y1, pitch;
     bool on;
@@ -188,19 +185,19 @@ struct Searcher {
     const PicDev &R;
     uint32_t o[Gm::U];    // current block, packed bytes per unit
     int urow[Gm::U], ucol[Gm::U];
-    int roff[Gm::U];      // unit offset in the reference plane
     int grp;              // candidate group (N == 8)
     int o8;               // N == 8: this lane's current-block pixel
     gptr32 cplane;        // current picture
     size_t cblk;          // byte offset of the current block in it
     int cpitch;
     gptr32 rplane;        // reference picture
+    gptr64 aplane;        // reference anchors
     Win W;                // staged reference window (W.on == false: none)
     Counts cnt;
 
     __device__ Searcher(const PicDev &ref, const PicDev &cur, int px, int py, const Win &win)
         : R(ref), cplane(as_global32(cur.y)), cblk(static_cast<size_t>(py) * cur.pitch + px),
-          cpitch(cur.pitch), rplane(as_global32(ref.y)), W(win)
+          cpitch(cur.pitch), rplane(as_global32(ref.y)), aplane(as_global64(ref.an)), W(win)
     {
         const int lane = threadIdx.x & 63;
         grp = Gm::G == 1 ? 0 : lane / Gm::UNITS;
@@ -210,11 +207,12 @@ struct Searcher {
             urow[u] = unit / Gm::DW;
             ucol[u] = (unit % Gm::DW) * 4;
             o[u] = ld4g(cplane, cblk + static_cast<size_t>(urow[u]) * cpitch + ucol[u]);
-            roff[u] = urow[u] * R.pitch + ucol[u];
         }
         o8 = 0;
-        if (N == 8)
-            o8 = cur.y[static_cast<size_t>(py + (lane >> 3)) * cur.pitch + px + (lane & 7)];
+        if (N == 8) {
+            const uint32_t w = ld4g(cplane, cblk + static_cast<size_t>(lane >> 3) * cpitch + (lane & 4));
+            o8 = (w >> (8 * (lane & 3))) & 0xFF;
+        }
     }
 
     __device__ __forceinline__ bool in_win(int x, int y) const
@@ -222,16 +220,12 @@ struct Searcher {
         return W.on && x >= W.x0 && y >= W.y0 && x + N <= W.x1 && y + N <= W.y1;
     }
 
-    // Packed SAD of this lane's unit u for the block at (x,y): from the LDS
-    // window when the block lies inside it, else from the reference plane.
-    __device__ __forceinline__ uint32_t unit_sad(int x, int y, int u, bool lds, uint32_t acc) const
+    // Reference dword of unit u for the block at (x,y): from the LDS window
+    // when the block lies inside it, else from the reference plane.
+    __device__ __forceinline__ uint32_t ref4(int x, int y, int u, bool lds) const
     {
-        if (lds) {
-            const int b = (y - W.y0) * W.pitch + (x - W.x0);
-            return __builtin_amdgcn_sad_u8(ld4s(b + urow[u] * W.pitch + ucol[u]), o[u], acc);
-        }
-        const size_t b = static_cast<size_t>(y) * R.pitch + x;
-        return __builtin_amdgcn_sad_u8(ld4g(rplane, b + roff[u]), o[u], acc);
+        if (lds) return ld4s((y - W.y0 + urow[u]) * W.pitch + (x - W.x0) + ucol[u]);
+        return ld4g(rplane, static_cast<size_t>(y + urow[u]) * R.pitch + x + ucol[u]);
     }
 
     // SAD of the block at full-pel (x,y), which must fit in the frame.
@@ -240,68 +234,84 @@ struct Searcher {
         const bool lds = in_win(x, y);
         if (Gm::G == 1) {
             uint32_t acc = 0;
-            if (lds) {
 #pragma unroll
-                for (int u = 0; u < Gm::U; ++u) acc = unit_sad(x, y, u, true, acc);
-            } else {
-#pragma unroll
-                for (int u = 0; u < Gm::U; ++u) acc = unit_sad(x, y, u, false, acc);
-            }
+            for (int u = 0; u < Gm::U; ++u) acc = __builtin_amdgcn_sad_u8(ref4(x, y, u, lds), o[u], acc);
             return wave_sum(static_cast<int>(acc));
-        } else {
-            const int v = row_scan(static_cast<int>(unit_sad(x, y, 0, lds, 0u)));
-            return __builtin_amdgcn_readlane(v, 15);  // group 0 priced it; all groups did
+        }
+        const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ref4(x, y, 0, lds), o[0], 0u)));
+        return __builtin_amdgcn_readlane(v, 15);  // group 0 priced it; all groups did
+    }
+
+    // N >= 16: prices candidates first .. first+B-1 of pattern P (those < K)
+    // around (bx,by); all their loads are issued before the first SAD so the
+    // batch costs one memory latency.  Off-frame ones get kInf.
+    template <int P>
+    __device__ __forceinline__ void price_batch(int bx, int by, int scale, int first,
+                                                int32_t (&s)[Gm::B]) const
+    {
+        constexpr int K = Pat<P>::K;
+        uint32_t v[Gm::B][Gm::U];
+        bool ok[Gm::B];
+#pragma unroll
+        for (int b = 0; b < Gm::B; ++b) {
+            const int k = first + b;
+            const int x = bx + Pat<P>::x(k) * scale, y = by + Pat<P>::y(k) * scale;
+            ok[b] = k < K && fits(R, N, x, y);
+            // a rejected candidate reads the (in-frame) centre instead
+            const int lx = ok[b] ? x : bx, ly = ok[b] ? y : by;
+            const bool lds = in_win(lx, ly);
+#pragma unroll
+            for (int u = 0; u < Gm::U; ++u) v[b][u] = ref4(lx, ly, u, lds);
+        }
+#pragma unroll
+        for (int b = 0; b < Gm::B; ++b) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int u = 0; u < Gm::U; ++u) acc = __builtin_amdgcn_sad_u8(v[b][u], o[u], acc);
+            const int t = wave_sum(static_cast<int>(acc));
+            s[b] = ok[b] ? t : kInf;
         }
     }
 
-    // Prices candidates k >= first of a K-point set; others get kInf.
+    // N == 8: prices candidates k >= first of a K-point set, four per DPP
+    // row group; others get kInf.
     template <int K>
-    __device__ __forceinline__ void price_int(const int (&xs)[K], const int (&ys)[K], int first,
-                                              int32_t (&s)[K]) const
+    __device__ __forceinline__ void price_grouped(const int (&xs)[K], const int (&ys)[K], int first,
+                                                  int32_t (&s)[K]) const
     {
-        if (Gm::G == 1) {
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-                s[k] = (k >= first && fits(R, N, xs[k], ys[k])) ? sad1(xs[k], ys[k]) : kInf;
-        } else {
+        for (int s0 = 0; s0 < K; s0 += 4) {
+            if (s0 + 3 < first) {
 #pragma unroll
-            for (int s0 = 0; s0 < K; s0 += 4) {
-                if (s0 + 3 < first) {
+                for (int g = 0; g < 4; ++g)
+                    if (s0 + g < K) s[s0 + g] = kInf;
+                continue;
+            }
+            int x = 0, y = 0;
+            bool live = false;
 #pragma unroll
-                    for (int g = 0; g < 4; ++g)
-                        if (s0 + g < K) s[s0 + g] = kInf;
-                    continue;
+            for (int g = 0; g < 4; ++g) {
+                const int k = s0 + g;
+                if (k < K && grp == g) {
+                    live = k >= first && fits(R, N, xs[k], ys[k]);
+                    x = xs[k];
+                    y = ys[k];
                 }
-                int x = 0, y = 0;
-                bool live = false;
+            }
+            if (!live) { x = 0; y = 0; }
+            const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ref4(x, y, 0, in_win(x, y)), o[0], 0u)));
 #pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const int k = s0 + g;
-                    if (k < K && grp == g) {
-                        live = k >= first && fits(R, N, xs[k], ys[k]);
-                        x = xs[k];
-                        y = ys[k];
-                    }
-                }
-                if (!live) { x = 0; y = 0; }
-                const int v = row_scan(static_cast<int>(unit_sad(x, y, 0, in_win(x, y), 0u)));
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const int k = s0 + g;
-                    if (k < K) {
-                        const int r = __builtin_amdgcn_readlane(v, 16 * g + 15);
-                        s[k] = (k >= first && fits(R, N, xs[k], ys[k])) ? r : kInf;
-                    }
+            for (int g = 0; g < 4; ++g) {
+                const int k = s0 + g;
+                if (k < K) {
+                    const int r = __builtin_amdgcn_readlane(v, 16 * g + 15);
+                    s[k] = (k >= first && fits(R, N, xs[k], ys[k])) ? r : kInf;
                 }
             }
         }
     }
 
     // One greedy integer pattern stage (reference src/motion.c:399-448).
-    // N >= 16: one candidate occupies the whole wave, so candidates are priced
-    // in the reference's own order (no speculation, no wasted SADs).
-    // N == 8: the four DPP rows price four candidates at once; the first
-    // strict improvement in table order is taken and the rest re-priced.
     template <int P>
     __device__ __forceinline__ void stage(int scale, int passes, int &bx, int &by, int32_t &best)
     {
@@ -310,13 +320,33 @@ struct Searcher {
 #pragma unroll 1
             for (int it = 0; it < passes; ++it) {
                 bool moved = false;
+                int first = 0;
 #pragma unroll 1
-                for (int k = 0; k < K; ++k) {
-                    const int x = bx + Pat<P>::x(k) * scale, y = by + Pat<P>::y(k) * scale;
-                    int32_t sv = kInf;
-                    if (fits(R, N, x, y)) { sv = sad1(x, y); cnt.ie++; }
-                    else cnt.oob++;
-                    if (sv < best) { best = sv; bx = x; by = y; moved = true; }
+                while (first < K) {
+                    int32_t s[Gm::B];
+                    price_batch<P>(bx, by, scale, first, s);
+                    int hit = -1;
+#pragma unroll
+                    for (int b = 0; b < Gm::B; ++b) {
+                        const int k = first + b;
+                        if (k < K && hit < 0) {
+                            // the sequential loop reaches candidate k: count it
+                            const int x = bx + Pat<P>::x(k) * scale, y = by + Pat<P>::y(k) * scale;
+                            if (fits(R, N, x, y)) cnt.ie++; else cnt.oob++;
+                            if (s[b] < best) hit = b;
+                        }
+                    }
+                    if (hit < 0) { first += Gm::B; continue; }
+                    int32_t nb = kInf;
+#pragma unroll
+                    for (int b = 0; b < Gm::B; ++b)
+                        if (b == hit) nb = s[b];
+                    const int k = first + hit;
+                    bx += Pat<P>::x(k) * scale;
+                    by += Pat<P>::y(k) * scale;
+                    best = nb;
+                    moved = true;
+                    first = k + 1;
                 }
                 if (!moved) break;
             }
@@ -334,13 +364,12 @@ struct Searcher {
                     xs[k] = bx + Pat<P>::x(k) * scale;
                     ys[k] = by + Pat<P>::y(k) * scale;
                 }
-                price_int<K>(xs, ys, first, s);
+                price_grouped<K>(xs, ys, first, s);
                 int hit = K, nx = bx, ny = by;
                 int32_t nb = best;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     if (k >= first && hit == K) {
-                        // the sequential loop reaches candidate k: count it
                         if (fits(R, N, xs[k], ys[k])) cnt.ie++; else cnt.oob++;
                         if (s[k] < best) { hit = k; nb = s[k]; nx = xs[k]; ny = ys[k]; }
                     }
@@ -353,21 +382,25 @@ struct Searcher {
         }
     }
 
-    // SAD of the quarter-pel candidate (qx,qy) of class cls (1 or 2).
-    __device__ __forceinline__ int32_t qsad(int qx, int qy, int cls) const
+    // ---- quarter-pel --------------------------------------------------------
+
+    // SAD of a quarter-pel candidate read directly from the planes (edge path).
+    __device__ __forceinline__ int32_t qsad_direct(int qx, int qy, int cls) const
     {
         const int fx = qx >> 2, fy = qy >> 2, dx = qx & 3, dy = qy & 3;
+        const int lane = threadIdx.x & 63;
         if (N == 8) {
-            const int lane = threadIdx.x & 63;
             const int x = fx + (lane & 7), y = fy + (lane >> 3);
-            const int v = cls == 1 ? q6_px(R, x, y, dx, dy) : bil_px(R, x, y, dx, dy);
+            int v;
+            if (cls == 1) {
+                const size_t e = static_cast<size_t>(y) * R.apitch + x;
+                v = qpix(aplane[e], aplane[e + 1], aplane[e + R.apitch], aplane[e + R.apitch + 1], dx, dy);
+            } else {
+                v = bil_px(R, x, y, dx, dy);
+            }
             const int d = o8 - v;
             return wave_sum(d < 0 ? -d : d);
         }
-        // Units are walked with a rolled loop (registers stay bounded for
-        // N = 64); the current block is re-read from its plane (cache hits)
-        // so no register array is indexed dynamically.
-        const int lane = threadIdx.x & 63;
         uint32_t acc = 0;
 #pragma unroll 1
         for (int u = 0; u < Gm::U; ++u) {
@@ -377,7 +410,13 @@ struct Searcher {
             uint32_t pk = 0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int v = cls == 1 ? q6_px(R, x + j, y, dx, dy) : bil_px(R, x + j, y, dx, dy);
+                int v;
+                if (cls == 1) {
+                    const size_t e = static_cast<size_t>(y) * R.apitch + x + j;
+                    v = qpix(aplane[e], aplane[e + 1], aplane[e + R.apitch], aplane[e + R.apitch + 1], dx, dy);
+                } else {
+                    v = bil_px(R, x + j, y, dx, dy);
+                }
                 pk |= static_cast<uint32_t>(v) << (8 * j);
             }
             acc = __builtin_amdgcn_sad_u8(pk, ld4g(cplane, cblk + static_cast<size_t>(ur) * cpitch + uc), acc);
@@ -385,19 +424,141 @@ struct Searcher {
         return wave_sum(static_cast<int>(acc));
     }
 
+    // Packed SAD of one unit (4 pixels) for a candidate whose anchor rows are
+    // ra (sample row) and rb (row below), starting at column ox.
+    template <int C>
+    __device__ __forceinline__ uint32_t unit_q(const anc_t (&ra)[C], const anc_t (&rb)[C], int ox,
+                                               int dx, int dy, uint32_t orig, uint32_t acc) const
+    {
+        uint32_t pk = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const anc_t p00 = ox ? ra[j + 1] : ra[j], p10 = ox ? ra[j + 2] : ra[j + 1];
+            const anc_t p01 = ox ? rb[j + 1] : rb[j], p11 = ox ? rb[j + 2] : rb[j + 1];
+            pk |= static_cast<uint32_t>(qpix(p00, p10, p01, p11, dx, dy)) << (8 * j);
+        }
+        return __builtin_amdgcn_sad_u8(pk, orig, acc);
+    }
+
     // Quarter-pel refinement, one greedy pass (reference src/motion.c:450-481).
-    // Each candidate uses the whole wave, so they are priced in order.
-    __device__ __forceinline__ void qpel(int &qx, int &qy, int32_t &best)
+    // (bx,by) is the integer best.  When every sample position the pass can
+    // reach (fx in {bx-1, bx}, fy in {by-1, by}) is six-tap safe, the anchor
+    // neighbourhood is loaded once: N <= 16 keep it in registers and price
+    // candidates in the reference's order; N >= 32 price all remaining
+    // candidates per unit in rounds (first strict improvement wins, the rest
+    // are re-priced).  Otherwise candidates are priced one by one from the
+    // planes with their own class.
+    __device__ __forceinline__ void qpel(int bx, int by, int &qx, int &qy, int32_t &best)
     {
         constexpr int K = Pat<PAT_QPEL>::K;
+        qx = bx * 4;
+        qy = by * 4;
+        const bool fast = bx - 1 >= 2 && by - 1 >= 2 && bx + N + 2 < R.w && by + N + 2 < R.h;
+        if (!fast) {
 #pragma unroll 1
-        for (int k = 0; k < K; ++k) {
-            const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
-            const int cls = qclass(R, N, tx, ty);
-            if (!cls) continue;
-            if (cls == 1) cnt.q6++; else cnt.qb++;
-            const int32_t sv = qsad(tx, ty, cls);
-            if (sv < best) { best = sv; qx = tx; qy = ty; }
+            for (int k = 0; k < K; ++k) {
+                const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
+                const int cls = qclass(R, N, tx, ty);
+                if (!cls) continue;
+                if (cls == 1) cnt.q6++; else cnt.qb++;
+                const int32_t sv = qsad_direct(tx, ty, cls);
+                if (sv < best) { best = sv; qx = tx; qy = ty; }
+            }
+            return;
+        }
+        const int lane = threadIdx.x & 63;
+        const int x0 = bx - 1, y0 = by - 1;  // window origin (sample coordinates)
+        if (N == 8) {
+            // lane = pixel; a 3x3 anchor window in registers
+            const int c = lane & 7, r = lane >> 3;
+            anc_t e[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    e[i][j] = aplane[static_cast<size_t>(y0 + r + i) * R.apitch + x0 + c + j];
+#pragma unroll 1
+            for (int k = 0; k < K; ++k) {
+                const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
+                const int ox = (tx >> 2) - x0, oy = (ty >> 2) - y0;
+                const int dx = tx & 3, dy = ty & 3;
+                cnt.q6++;
+                const anc_t p00 = oy ? (ox ? e[1][1] : e[1][0]) : (ox ? e[0][1] : e[0][0]);
+                const anc_t p10 = oy ? (ox ? e[1][2] : e[1][1]) : (ox ? e[0][2] : e[0][1]);
+                const anc_t p01 = oy ? (ox ? e[2][1] : e[2][0]) : (ox ? e[1][1] : e[1][0]);
+                const anc_t p11 = oy ? (ox ? e[2][2] : e[2][1]) : (ox ? e[1][2] : e[1][1]);
+                const int d = o8 - qpix(p00, p10, p01, p11, dx, dy);
+                const int32_t sv = wave_sum(d < 0 ? -d : d);
+                if (sv < best) { best = sv; qx = tx; qy = ty; }
+            }
+            return;
+        }
+        if (Gm::U == 1) {
+            // N == 16: one unit per lane; a 3x6 anchor window in registers
+            anc_t e[3][6];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+                    e[i][j] = aplane[static_cast<size_t>(y0 + urow[0] + i) * R.apitch + x0 + ucol[0] + j];
+#pragma unroll 1
+            for (int k = 0; k < K; ++k) {
+                const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
+                const int ox = (tx >> 2) - x0, oy = (ty >> 2) - y0;
+                cnt.q6++;
+                uint32_t acc;
+                if (oy) acc = unit_q<6>(e[1], e[2], ox, tx & 3, ty & 3, o[0], 0u);
+                else acc = unit_q<6>(e[0], e[1], ox, tx & 3, ty & 3, o[0], 0u);
+                const int32_t sv = wave_sum(static_cast<int>(acc));
+                if (sv < best) { best = sv; qx = tx; qy = ty; }
+            }
+            return;
+        }
+        // N >= 32: rounds over the units, all remaining candidates per unit
+        int first = 0;
+#pragma unroll 1
+        while (first < K) {
+            uint32_t acc[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc[k] = 0;
+#pragma unroll 1
+            for (int u = 0; u < Gm::U; ++u) {
+                const int unit = lane + 64 * u;
+                const int ur = unit / Gm::DW, uc = (unit % Gm::DW) * 4;
+                anc_t e[3][6];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 6; ++j)
+                        e[i][j] = aplane[static_cast<size_t>(y0 + ur + i) * R.apitch + x0 + uc + j];
+                const uint32_t orig = ld4g(cplane, cblk + static_cast<size_t>(ur) * cpitch + uc);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if (k < first) continue;
+                    const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
+                    const int ox = (tx >> 2) - x0, oy = (ty >> 2) - y0;
+                    if (oy) acc[k] = unit_q<6>(e[1], e[2], ox, tx & 3, ty & 3, orig, acc[k]);
+                    else acc[k] = unit_q<6>(e[0], e[1], ox, tx & 3, ty & 3, orig, acc[k]);
+                }
+            }
+            int hit = K, nx = qx, ny = qy;
+            int32_t nb = best;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (k >= first && hit == K) {
+                    cnt.q6++;
+                    const int32_t sv = wave_sum(static_cast<int>(acc[k]));
+                    if (sv < best) {
+                        hit = k;
+                        nb = sv;
+                        nx = qx + Pat<PAT_QPEL>::x(k);
+                        ny = qy + Pat<PAT_QPEL>::y(k);
+                    }
+                }
+            }
+            if (hit == K) break;
+            best = nb; qx = nx; qy = ny;
+            first = hit + 1;
         }
     }
 };
@@ -419,9 +580,7 @@ __device__ void search(const PicDev &R, const PicDev &C, const tcme_call &c, tcm
         S.template stage<PAT_LARGE>(scale, 2, bx, by, best);
         S.template stage<PAT_SMALL>(1, 3, bx, by, best);
         S.template stage<PAT_DIAMOND>(1, 4, bx, by, best);
-        qx = bx * 4;
-        qy = by * 4;
-        S.qpel(qx, qy, best);
+        S.qpel(bx, by, qx, qy, best);
     } else {
         // No candidate can fit: one pass of each stage, all rejected; every
         // quarter-pel candidate fails the frame test.
@@ -452,7 +611,7 @@ __device__ __forceinline__ int check_call(const PicDev *table, int npics, const 
     if (c.blk != 8 && c.blk != 16 && c.blk != 32 && c.blk != 64) return TCME_E_BLK;
     R = table[c.ref_pic];
     C = table[c.cur_pic];
-    if (!R.y || !R.hh || !C.y) return TCME_E_PIC;  // empty slot, or no half-sample planes
+    if (!R.y || !R.an || !C.y) return TCME_E_PIC;  // empty slot, or a reference without anchors
     if (c.px < 0 || c.py < 0 || c.px + c.blk > C.w || c.py + c.blk > C.h) return TCME_E_ARG;
     return 0;
 }
@@ -475,14 +634,23 @@ __device__ __forceinline__ void run_call(const PicDev *table, int npics, const t
         return;
     }
     switch (c.blk) {
+#ifndef TCME_ONLY_N
     case 8: search<8>(R, C, c, res + i, st, win); break;
     case 16: search<16>(R, C, c, res + i, st, win); break;
     case 32: search<32>(R, C, c, res + i, st, win); break;
     default: search<64>(R, C, c, res + i, st, win); break;
+#else
+    default: search<TCME_ONLY_N>(R, C, c, res + i, st, win); break;
+#endif
     }
 }
 
-// One wavefront per call, no staging (the single-call drop-in).
+// One wavefront per call, block size N only: the plain form launches one
+// instance per size (their register budgets differ 4x: an 8x8 search needs
+// about a quarter of a 64x64 one, so small blocks run at full occupancy).
+// A wave whose call has another size exits at once; the 64 instance also
+// answers descriptors with an unsupported size.
+template <int N>
 __global__ __launch_bounds__(64) void me_search_kernel(const PicDev *__restrict__ table, int npics,
                                                         const tcme_call *__restrict__ calls, int n,
                                                         tcme_result *__restrict__ res,
@@ -490,8 +658,24 @@ __global__ __launch_bounds__(64) void me_search_kernel(const PicDev *__restrict_
 {
     const int i = blockIdx.x;
     if (i >= n) return;
+    const tcme_call c = calls[i];
+    const bool mine = c.blk == N || (N == 64 && c.blk != 8 && c.blk != 16 && c.blk != 32);
+    if (!mine) return;
+    PicDev R{}, C{};
+    const int status = check_call(table, npics, c, R, C);
+    tcme_stats *st = stats ? stats + i : nullptr;
+    if (status) {
+        if (threadIdx.x == 0) {
+            res[i].mv_x = 0;
+            res[i].mv_y = 0;
+            res[i].sad = kInf;
+            res[i].status = status;
+            if (st) { st->int_evals = st->q6_evals = st->qbil_evals = st->int_oob = 0; }
+        }
+        return;
+    }
     const Win none{0, 0, 0, 0, 0, false};
-    run_call(table, npics, calls[i], i, res, stats, none);
+    search<N>(R, C, c, res + i, st, none);
 }
 
 // Block index -> chunk index so that each of the 8 XCDs (which receive
@@ -514,12 +698,12 @@ __device__ __forceinline__ int call_cost(int blk)
 // Chunked search: a workgroup of WAVES wavefronts takes `chunk` consecutive
 // calls and stages the union of their reference windows (clamped centre +-
 // apron, one reference picture) in LDS with coalesced dword loads.  Each
-// wavefront then takes a contiguous, cost-balanced share of the chunk (calls
-// ordered largest first split evenly) and prices candidates from LDS; a
-// candidate outside the window reads the reference plane instead, so the
-// apron only affects speed, never results.  All control flow below the
-// barriers is wave-uniform: the wavefront index comes from readfirstlane and
-// the share from scalar loads of the call list.
+// wavefront then takes a contiguous, cost-balanced share of the chunk and
+// prices integer candidates from LDS; a candidate outside the window reads
+// the reference plane instead, so the apron only affects speed, never
+// results.  All control flow below the barriers is wave-uniform: the
+// wavefront index comes from readfirstlane and the share from scalar loads of
+// the call list.
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(
     const PicDev *__restrict__ table, int npics, const tcme_call *__restrict__ calls, int n,
@@ -591,12 +775,12 @@ __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(
     }
 }
 
-// Raw six-tap half-sample planes (reference src/motion.c:70-107), one thread
-// per sample; positions whose taps would leave the frame are written as 0.
-__global__ __launch_bounds__(256) void halfpel_kernel(const uint8_t *__restrict__ y, int pitch,
-                                                      int w, int h, int16_t *__restrict__ hh,
-                                                      int16_t *__restrict__ vv,
-                                                      int16_t *__restrict__ cc, int hpitch)
+// Anchor plane of a picture (reference src/motion.c:70-107 for the taps):
+// one thread per sample writes {sample, hh, vv, cc}; half samples whose taps
+// would leave the frame are written as 0 and never read by the search.
+__global__ __launch_bounds__(256) void anchor_kernel(const uint8_t *__restrict__ y, int pitch,
+                                                     int w, int h, uint64_t *__restrict__ an,
+                                                     int apitch)
 {
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int r = blockIdx.y * 4 + threadIdx.y;
@@ -619,29 +803,39 @@ __global__ __launch_bounds__(256) void halfpel_kernel(const uint8_t *__restrict_
         }
         cval = tap6(t[0], t[1], t[2], t[3], t[4], t[5]);
     }
-    const size_t o = static_cast<size_t>(r) * hpitch + x;
-    hh[o] = static_cast<int16_t>(hv);
-    vv[o] = static_cast<int16_t>(vval);
-    cc[o] = static_cast<int16_t>(cval);
+    const uint32_t lo = static_cast<uint32_t>(row[x]) | (static_cast<uint32_t>(hv) << 16);
+    const uint32_t hi = (static_cast<uint32_t>(vval) & 0xFFFFu) | (static_cast<uint32_t>(cval) << 16);
+    an[static_cast<size_t>(r) * apitch + x] = static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
 }
 
 }  // namespace
 
-hipError_t launch_halfpel(const PicDev &p, hipStream_t s)
+hipError_t launch_anchors(const PicDev &p, hipStream_t s)
 {
     dim3 grid((p.w + 63) / 64, (p.h + 3) / 4), block(64, 4);
-    hipLaunchKernelGGL(halfpel_kernel, grid, block, 0, s, p.y, p.pitch, p.w, p.h,
-                       const_cast<int16_t *>(p.hh), const_cast<int16_t *>(p.vv),
-                       const_cast<int16_t *>(p.cc), p.hpitch);
+    hipLaunchKernelGGL(anchor_kernel, grid, block, 0, s, p.y, p.pitch, p.w, p.h,
+                       const_cast<uint64_t *>(p.an), p.apitch);
     return hipGetLastError();
 }
 
-hipError_t launch_search(const PicDev *table, int npics, const tcme_call *calls, int n,
-                         tcme_result *res, tcme_stats *stats, hipStream_t s)
+hipError_t launch_search_n(int blk, const PicDev *table, int npics, const tcme_call *calls, int n,
+                           tcme_result *res, tcme_stats *stats, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(me_search_kernel, dim3(n), dim3(64), 0, s, table, npics, calls, n, res,
-                       stats);
+    switch (blk) {
+    case 8:
+        hipLaunchKernelGGL(me_search_kernel<8>, dim3(n), dim3(64), 0, s, table, npics, calls, n, res, stats);
+        break;
+    case 16:
+        hipLaunchKernelGGL(me_search_kernel<16>, dim3(n), dim3(64), 0, s, table, npics, calls, n, res, stats);
+        break;
+    case 32:
+        hipLaunchKernelGGL(me_search_kernel<32>, dim3(n), dim3(64), 0, s, table, npics, calls, n, res, stats);
+        break;
+    default:
+        hipLaunchKernelGGL(me_search_kernel<64>, dim3(n), dim3(64), 0, s, table, npics, calls, n, res, stats);
+        break;
+    }
     return hipGetLastError();
 }
 
@@ -650,8 +844,6 @@ hipError_t launch_search_tiled(const PicDev *table, int npics, const tcme_call *
                                hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    if (t.chunk == 1 && t.lds_bytes == 0)  // the plain form: one wavefront per call
-        return launch_search(table, npics, calls, n, res, stats, s);
     const int nb = (n + t.chunk - 1) / t.chunk;
     switch (t.waves) {
     case 1:
