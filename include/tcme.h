@@ -100,12 +100,11 @@ int  tcme_pic_finish(tcme_ctx *ctx, int pic, void *stream);
 int  tcme_search(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,
                  tcme_stats *stats, void *stream);
 
-/* Launch shape of tcme_search: calls per workgroup (a workgroup stages the
- * union of its calls' reference windows in LDS), wavefronts per workgroup
- * (1, 2, 4 or 8), window apron in pixels around each clamped centre, and LDS
- * bytes per workgroup (<= 65536).  Results never depend on it. */
+/* Launch shape of tcme_search: wavefronts cooperating on one call, per
+ * block size (8x8: 1; 16x16: 1; 32x32: 1, 2 or 4; 64x64: 2, 4 or 8).
+ * Results never depend on it. */
 typedef struct tcme_tiling {
-    int32_t chunk, waves, apron, lds_bytes;
+    int32_t waves8, waves16, waves32, waves64;
 } tcme_tiling;
 
 int  tcme_set_tiling(tcme_ctx *ctx, const tcme_tiling *t);

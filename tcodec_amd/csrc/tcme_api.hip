@@ -1,9 +1,8 @@
 // Host runtime and C ABI of libtcme.so (declared in include/tcme.h).
 //
 // Batch API: a context owns a device picture table; pictures are luma planes
-// with their half-sample planes; tcme_search() launches the search in the
-// context's launch shape (plain: one wavefront per call; chunked: LDS-staged
-// reference windows shared by a workgroup's calls).
+// with their anchor planes; tcme_search() launches the four per-size search
+// kernels in the context's launch shape.
 //
 // Drop-in: tc_motion_est() keeps device mirrors of the reference planes it is
 // handed.  Before each call it compares, byte for byte, the part of the host
@@ -152,22 +151,21 @@ int pic_free(tcme_ctx *c, int id)
     return TCME_OK;
 }
 
-// Launches the search of n calls in launch shape t on stream s.
+// Launches the search of n calls in launch shape t on stream s: the four
+// per-size kernels run concurrently on the context's streams, forked from and
+// joined back into s with events.
 int dispatch(tcme_ctx *c, const tcme_call *calls, int n, const tcme::Tiling &t,
              tcme_result *res, tcme_stats *stats, hipStream_t s)
 {
     if (n <= 0) return TCME_OK;
-    if (t.chunk > 1 || t.lds_bytes > 0) {
-        HIP_TRY(tcme::launch_search_tiled(c->d_table, c->max_pics, calls, n, t, res, stats, s));
-        return TCME_OK;
-    }
     std::lock_guard<std::mutex> g(c->launch_mu);
     static const int sizes[4] = {64, 32, 16, 8};
+    static const int slot[4] = {3, 2, 1, 0};
     HIP_TRY(hipEventRecord(c->fork, s));
     for (int k = 0; k < 4; ++k) {
         HIP_TRY(hipStreamWaitEvent(c->sub[k], c->fork, 0));
-        HIP_TRY(tcme::launch_search_n(sizes[k], c->d_table, c->max_pics, calls, n, res, stats,
-                                      c->sub[k]));
+        HIP_TRY(tcme::launch_search_n(sizes[k], t.waves[slot[k]], c->d_table, c->max_pics, calls,
+                                      n, res, stats, c->sub[k]));
         HIP_TRY(hipEventRecord(c->join[k], c->sub[k]));
         HIP_TRY(hipStreamWaitEvent(s, c->join[k], 0));
     }
@@ -310,22 +308,20 @@ namespace {
 int check_tiling(const tcme_tiling *t)
 {
     if (!t) return fail(TCME_E_ARG, "null tiling");
-    if (t->chunk < 1 || t->chunk > 4096) return fail(TCME_E_ARG, "chunk %d out of [1,4096]", t->chunk);
-    if (t->waves != 1 && t->waves != 2 && t->waves != 4 && t->waves != 8)
-        return fail(TCME_E_ARG, "waves %d not in {1,2,4,8}", t->waves);
-    if (t->apron < 0 || t->apron > 512) return fail(TCME_E_ARG, "apron %d out of [0,512]", t->apron);
-    if (t->lds_bytes < 0 || t->lds_bytes > 64 * 1024)
-        return fail(TCME_E_ARG, "lds_bytes %d out of [0,65536]", t->lds_bytes);
+    const int w[4] = {t->waves8, t->waves16, t->waves32, t->waves64};
+    for (int k = 0; k < 4; ++k)
+        if (!tcme::valid_waves(8 << k, w[k]))
+            return fail(TCME_E_ARG, "%d waves per %dx%d call not supported", w[k], 8 << k, 8 << k);
     return TCME_OK;
 }
 
 tcme::Tiling to_internal(const tcme_tiling &t)
 {
     tcme::Tiling r;
-    r.chunk = t.chunk;
-    r.waves = t.waves;
-    r.apron = t.apron;
-    r.lds_bytes = t.lds_bytes;
+    r.waves[0] = t.waves8;
+    r.waves[1] = t.waves16;
+    r.waves[2] = t.waves32;
+    r.waves[3] = t.waves64;
     return r;
 }
 
@@ -342,16 +338,17 @@ TCME_API int tcme_set_tiling(tcme_ctx *c, const tcme_tiling *t)
 TCME_API int tcme_get_tiling(tcme_ctx *c, tcme_tiling *t)
 {
     if (!c || !t) return fail(TCME_E_ARG, "null argument");
-    t->chunk = c->tiling.chunk;
-    t->waves = c->tiling.waves;
-    t->apron = c->tiling.apron;
-    t->lds_bytes = c->tiling.lds_bytes;
+    t->waves8 = c->tiling.waves[0];
+    t->waves16 = c->tiling.waves[1];
+    t->waves32 = c->tiling.waves[2];
+    t->waves64 = c->tiling.waves[3];
     return TCME_OK;
 }
 
-// Times every launch shape of a small grid on the caller's call list (device
-// arrays; results are overwritten) and keeps the fastest in the context.
-// Results do not depend on the shape, only time does.
+// Times every launch shape (wavefronts per 32x32 and per 64x64 call) on the
+// caller's call list (device arrays; results are overwritten with the same
+// answers) and keeps the fastest in the context.  Results do not depend on
+// the shape, only time does.
 TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_result *results,
                            int reps, void *stream, tcme_tiling *best_out, float *best_ms)
 {
@@ -360,47 +357,29 @@ TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_resu
     if (reps < 1) reps = 3;
     if (set_device(c) != TCME_OK) return TCME_E_HIP;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    static const int chunks[] = {43, 85, 170};
-    static const int waves[] = {2, 4, 8};
-    static const int aprons[] = {24, 40, 56};
+    static const int w32[] = {1, 2, 4};
+    static const int w64[] = {2, 4, 8};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    tcme::Tiling best;  // starts at the plain form, which is timed first
+    tcme::Tiling best;
     float best_t = 1e30f;
-    {
-        const tcme::Tiling plain;
-        if (int e = dispatch(c, calls, n, plain, results, nullptr, s)) return e;
-        HIP_TRY(hipEventRecord(e0, s));
-        for (int r = 0; r < reps; ++r)
-            if (int e = dispatch(c, calls, n, plain, results, nullptr, s)) return e;
-        HIP_TRY(hipEventRecord(e1, s));
-        HIP_TRY(hipEventSynchronize(e1));
-        HIP_TRY(hipEventElapsedTime(&best_t, e0, e1));
-        best_t /= reps;
-    }
-    for (int ch : chunks)
-        for (int wv : waves)
-            for (int ap : aprons) {
-                tcme::Tiling t;
-                t.chunk = ch;
-                t.waves = wv;
-                t.apron = ap;
-                const int span = 64 * ((ch + 84) / 85);
-                int lds = (span + 2 * ap + 12) * (64 + 2 * ap) + 64;
-                lds = (lds + 1023) / 1024 * 1024;
-                t.lds_bytes = lds > 64 * 1024 ? 64 * 1024 : lds;
-                if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;  // warm
-                HIP_TRY(hipEventRecord(e0, s));
-                for (int r = 0; r < reps; ++r)
-                    if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;
-                HIP_TRY(hipEventRecord(e1, s));
-                HIP_TRY(hipEventSynchronize(e1));
-                float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                ms /= reps;
-                if (ms < best_t) { best_t = ms; best = t; }
-            }
+    for (int a : w32)
+        for (int b : w64) {
+            tcme::Tiling t;
+            t.waves[2] = a;
+            t.waves[3] = b;
+            if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;  // warm
+            HIP_TRY(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r)
+                if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;
+            HIP_TRY(hipEventRecord(e1, s));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            if (ms < best_t) { best_t = ms; best = t; }
+        }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->tiling = best;
@@ -582,7 +561,9 @@ TCME_API tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, in
     if (hipMemcpyAsync(d.d_call, d.h_stage, sizeof call, hipMemcpyHostToDevice, d.stream) != hipSuccess ||
         hipMemcpy2DAsync(d.d_blk, d.blk_pitch, blk, n, n, n, hipMemcpyHostToDevice, d.stream) != hipSuccess)
         die("copy block");
-    if (tcme::launch_search_n(n, d.ctx->d_table, d.ctx->max_pics, d.d_call, 1, d.d_res, nullptr, d.stream) != hipSuccess)
+    const int waves = d.ctx->tiling.waves[n == 8 ? 0 : n == 16 ? 1 : n == 32 ? 2 : 3];
+    if (tcme::launch_search_n(n, waves, d.ctx->d_table, d.ctx->max_pics, d.d_call, 1, d.d_res,
+                              nullptr, d.stream) != hipSuccess)
         die("launch");
     if (hipMemcpyAsync(d.h_res, d.d_res, sizeof(tcme_result), hipMemcpyDeviceToHost, d.stream) != hipSuccess ||
         hipStreamSynchronize(d.stream) != hipSuccess)

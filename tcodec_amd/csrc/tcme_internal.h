@@ -25,24 +25,18 @@ constexpr int kTailSlack = 256;
 constexpr int kMaxBlk = 64;
 
 // Launch shape of the search (chosen by tcme_autotune or set by the caller):
-// calls per workgroup, wavefronts per workgroup, LDS window apron in pixels
-// and LDS bytes per workgroup.  chunk 1 with no LDS is the plain form, one
-// wavefront per call (me_search_kernel); it is the default.
+// wavefronts per call for each block size.  8x8 and 16x16: 1; 32x32: 1, 2
+// or 4; 64x64: 2, 4 or 8.
 struct Tiling {
-    int chunk = 1;
-    int waves = 1;
-    int apron = 0;
-    int lds_bytes = 0;
+    int waves[4] = {1, 1, 2, 4};  // for block sizes 8, 16, 32, 64
 };
 
+bool valid_waves(int blk, int waves);
 hipError_t launch_anchors(const PicDev &p, hipStream_t s);
-// One plain-form kernel (one wavefront per call) for the calls of block size
-// blk in the list; launch all four sizes to cover a mixed list.
-hipError_t launch_search_n(int blk, const PicDev *table, int npics, const tcme_call *calls, int n,
-                           tcme_result *res, tcme_stats *stats, hipStream_t s);
-// The chunked kernel (Tiling::chunk > 1 or LDS in use).
-hipError_t launch_search_tiled(const PicDev *table, int npics, const tcme_call *calls, int n,
-                               const Tiling &t, tcme_result *res, tcme_stats *stats,
-                               hipStream_t s);
+// The search kernel for the calls of block size blk in the list (the others
+// are skipped); launch all four sizes to cover a mixed list.
+hipError_t launch_search_n(int blk, int waves, const PicDev *table, int npics,
+                           const tcme_call *calls, int n, tcme_result *res, tcme_stats *stats,
+                           hipStream_t s);
 
 }  // namespace tcme

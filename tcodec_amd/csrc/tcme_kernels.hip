@@ -3,10 +3,10 @@
 //   anchor_kernel     per reference picture: every sample with its raw six-tap
 //                     half samples, packed 8 bytes per sample (reference
 //                     src/motion.c:70-107)
-//   me_search_kernel  the hierarchical hexagon search + quarter-pel refinement,
-//                     one wavefront per call (reference src/motion.c:384-485)
-//   me_tile_kernel    the same search for a chunk of calls per workgroup, with
-//                     the union of their reference windows staged in LDS
+//   me_search_kernel  the hierarchical hexagon search + quarter-pel refinement
+//                     (reference src/motion.c:384-485): one workgroup per call,
+//                     WPC wavefronts splitting its block, one instance per
+//                     block size
 //
 // The walk is the reference's sequential greedy walk.  Its decisions are
 // wave-uniform (scalar unit); lanes price candidates.  Candidates may be
@@ -16,10 +16,12 @@
 // computes, because the candidates before the first improvement saw the same
 // centre, so results are bit-identical whatever the batch size.
 //
-// Integer block costs: v_sad_u8 on packed bytes (4 |a-b| per lane-op) and a
-// DPP row scan.  Quarter-pel: every candidate of one call lies within one
-// sample of the integer best (fx in {X-1, X}, fy in {Y-1, Y}), so a lane loads
-// its 3-row anchor neighbourhood once and prices candidates from registers.
+// Integer block costs: v_sad_u8 on packed bytes (4 |a-b| per lane-op), a DPP
+// row scan per wavefront and, when several wavefronts share a block, an LDS
+// exchange.  Quarter-pel: every candidate of one call lies within one sample
+// of the integer best (fx in {X-1, X}, fy in {Y-1, Y}), so the call stages the
+// (N+2) x (N+2) anchor window once in LDS and prices each candidate with code
+// specialised for its phase.
 #include "tcme_internal.h"
 
 namespace tcme {
@@ -49,15 +51,6 @@ typedef uint64_t anc_t;
 __device__ __forceinline__ uint32_t ld4g(gptr32 base, size_t off)
 {
     gptr32 q = base + (off >> 2);
-    return __builtin_amdgcn_alignbyte(q[1], q[0], static_cast<uint32_t>(off & 3));
-}
-
-// The LDS reference window of the chunked kernel (dynamic shared memory).
-extern __shared__ __align__(16) uint32_t tcme_win[];
-
-__device__ __forceinline__ uint32_t ld4s(int off)
-{
-    const uint32_t *q = tcme_win + (off >> 2);
     return __builtin_amdgcn_alignbyte(q[1], q[0], static_cast<uint32_t>(off & 3));
 }
 
@@ -103,18 +96,6 @@ template <> struct Pat<PAT_QPEL> {
     __device__ static constexpr int y(int k) { return (k == 0 || k == 1) ? 0 : (k == 2 || k == 4 || k == 5) ? 1 : -1; }
 };
 
-// Lane geometry for an N x N block held as dword units (4 pixels each).
-// N >= 16: every lane owns U units of one candidate (G == 1); integer
-//          candidates are priced B at a time.
-// N == 8 : 16 units, so the wave's four DPP rows price four candidates (G == 4).
-template <int N> struct Geo {
-    static constexpr int DW = N / 4;
-    static constexpr int UNITS = N * DW;
-    static constexpr int U = UNITS >= 64 ? UNITS / 64 : 1;
-    static constexpr int G = UNITS >= 64 ? 1 : 64 / UNITS;
-    static constexpr int B = N >= 64 ? 2 : 6;
-};
-
 struct Counts { int ie = 0, oob = 0, q6 = 0, qb = 0; };
 
 __device__ __forceinline__ bool fits(const PicDev &R, int n, int x, int y)
@@ -129,27 +110,6 @@ __device__ __forceinline__ int a_i(anc_t e) { return static_cast<int>(static_cas
 __device__ __forceinline__ int a_h(anc_t e) { return static_cast<int>(static_cast<uint32_t>(e)) >> 16; }
 __device__ __forceinline__ int a_v(anc_t e) { return static_cast<int>(static_cast<uint32_t>(e >> 32) << 16) >> 16; }
 __device__ __forceinline__ int a_c(anc_t e) { return static_cast<int>(static_cast<uint32_t>(e >> 32)) >> 16; }
-
-// Quarter-sample value from the four anchor entries around it, raw
-// intermediates and a single final clip (reference src/motion.c:193-260).
-// Each anchor row is a lattice L0 (sample), L1 (half), L2 (next sample); a
-// phase p picks L0, avg(L0,L1), L1, avg(L1,L2); with a == b for even phases,
-// (a + b + 1) >> 1 == a, so one expression covers all four.
-__device__ __forceinline__ int lattice(int p, int l0, int l1, int l2)
-{
-    const int a = p >= 2 ? l1 : l0;
-    const int b = p == 1 ? l1 : (p == 3 ? l2 : a);
-    return (a + b + 1) >> 1;
-}
-
-__device__ __forceinline__ int qpix(anc_t p00, anc_t p10, anc_t p01, anc_t p11, int dx, int dy)
-{
-    const int top = lattice(dx, a_i(p00), a_h(p00), a_i(p10));
-    const int mid = lattice(dx, a_v(p00), a_c(p00), a_v(p10));
-    const int bot = lattice(dx, a_i(p01), a_h(p01), a_i(p11));
-    const int v = lattice(dy, top, mid, bot);
-    return v < 0 ? 0 : (v > 255 ? 255 : v);
-}
 
 // Bilinear quarter-pel sample, rounded after each pass
 // (reference src/motion.c:111-116, 330-336); never leaves [0,255].
@@ -171,39 +131,92 @@ __device__ __forceinline__ int qclass(const PicDev &R, int n, int qx, int qy)
     return 2;
 }
 
-// Part of one reference picture staged in LDS for a chunk of calls:
-// frame region [x0,x1) x [y0,y1), `pitch` bytes per row (x0 is a multiple of
-// 4; every row has 4 readable bytes past its end).
-struct Win {
-    int x0, y0, x1, y1, pitch;
-    bool on;
+
+// Quarter-sample value from the anchors around it for phase (DX, DY), raw
+// intermediates and a single final clip (reference src/motion.c:193-260).
+// Each anchor row is a lattice L0 (sample), L1 (half), L2 (next sample);
+// phase p picks L0, avg(L0,L1), L1 or avg(L1,L2), and the rows are blended
+// the same way by DY.  Only the anchors the phase uses are read.
+template <int P>
+__device__ __forceinline__ int lat(int l0, int l1, int l2)
+{
+    if (P == 0) return l0;
+    if (P == 1) return (l0 + l1 + 1) >> 1;
+    if (P == 2) return l1;
+    return (l1 + l2 + 1) >> 1;
+}
+
+template <int DX, int DY>
+__device__ __forceinline__ int qpix(const anc_t *e, int stride)
+{
+    // e points at the sample's entry; e[1] is right of it, e[stride] below
+    const anc_t p00 = e[0];
+    const anc_t p10 = (DX == 3) ? e[1] : p00;
+    int v;
+    if (DY == 0) {
+        v = lat<DX>(a_i(p00), a_h(p00), a_i(p10));
+    } else if (DY == 2) {
+        v = lat<DX>(a_v(p00), a_c(p00), a_v(p10));
+    } else if (DY == 1) {
+        v = (lat<DX>(a_i(p00), a_h(p00), a_i(p10)) + lat<DX>(a_v(p00), a_c(p00), a_v(p10)) + 1) >> 1;
+    } else {
+        const anc_t p01 = e[stride];
+        const anc_t p11 = (DX == 3) ? e[stride + 1] : p01;
+        v = (lat<DX>(a_v(p00), a_c(p00), a_v(p10)) + lat<DX>(a_i(p01), a_h(p01), a_i(p11)) + 1) >> 1;
+    }
+    if (DX == 0 && DY == 0) return v;
+    return v < 0 ? 0 : (v > 255 ? 255 : v);
+}
+
+// Lane geometry for an N x N block split over WPC wavefronts, held as dword
+// units (4 pixels each).  N >= 16: lane l of wavefront w owns units
+// (64 w + l) + 64 WPC u, u < U.  N == 8: 16 units, so the wavefront's four
+// DPP rows price four candidates (G == 4).  Integer candidates are priced B
+// at a time (all their loads issued before the first SAD).
+template <int N, int WPC> struct Geo {
+    static constexpr int DW = N / 4;
+    static constexpr int UNITS = N * DW;
+    static constexpr int U = UNITS >= 64 * WPC ? UNITS / (64 * WPC) : 1;
+    static constexpr int G = UNITS >= 64 ? 1 : 64 / UNITS;
+    static constexpr int B = U <= 2 ? 6 : (U <= 4 ? 3 : 2);
+    static constexpr int WIN = N + 2;  // quarter-pel anchor window side
 };
 
-template <int N>
+template <int N, int WPC>
+struct Shared {
+    anc_t win[(N + 2) * (N + 2)];  // quarter-pel anchor window
+    int red[2][WPC][8];            // cross-wavefront partial sums (double-buffered)
+};
+
+template <int N, int WPC>
 struct Searcher {
-    using Gm = Geo<N>;
+    using Gm = Geo<N, WPC>;
     const PicDev &R;
+    Shared<N, WPC> &sh;
     uint32_t o[Gm::U];    // current block, packed bytes per unit
     int urow[Gm::U], ucol[Gm::U];
+    int lane, wave;
     int grp;              // candidate group (N == 8)
     int o8;               // N == 8: this lane's current-block pixel
+    int phase;            // which half of sh.red the next exchange uses
     gptr32 cplane;        // current picture
     size_t cblk;          // byte offset of the current block in it
     int cpitch;
     gptr32 rplane;        // reference picture
     gptr64 aplane;        // reference anchors
-    Win W;                // staged reference window (W.on == false: none)
     Counts cnt;
 
-    __device__ Searcher(const PicDev &ref, const PicDev &cur, int px, int py, const Win &win)
-        : R(ref), cplane(as_global32(cur.y)), cblk(static_cast<size_t>(py) * cur.pitch + px),
-          cpitch(cur.pitch), rplane(as_global32(ref.y)), aplane(as_global64(ref.an)), W(win)
+    __device__ Searcher(const PicDev &ref, const PicDev &cur, int px, int py, Shared<N, WPC> &s)
+        : R(ref), sh(s), phase(0), cplane(as_global32(cur.y)),
+          cblk(static_cast<size_t>(py) * cur.pitch + px), cpitch(cur.pitch),
+          rplane(as_global32(ref.y)), aplane(as_global64(ref.an))
     {
-        const int lane = threadIdx.x & 63;
+        lane = threadIdx.x & 63;
+        wave = WPC == 1 ? 0 : __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
         grp = Gm::G == 1 ? 0 : lane / Gm::UNITS;
 #pragma unroll
         for (int u = 0; u < Gm::U; ++u) {
-            const int unit = Gm::G == 1 ? lane + 64 * u : lane % Gm::UNITS;
+            const int unit = Gm::G == 1 ? wave * 64 + lane + 64 * WPC * u : lane % Gm::UNITS;
             urow[u] = unit / Gm::DW;
             ucol[u] = (unit % Gm::DW) * 4;
             o[u] = ld4g(cplane, cblk + static_cast<size_t>(urow[u]) * cpitch + ucol[u]);
@@ -215,39 +228,53 @@ struct Searcher {
         }
     }
 
-    __device__ __forceinline__ bool in_win(int x, int y) const
+    // Whole-block sums of K wave-uniform partials: with several wavefronts
+    // per block the partials meet in LDS (one barrier); every wavefront gets
+    // the same totals, so all take the same walk.
+    template <int K>
+    __device__ __forceinline__ void block_sum(int32_t (&v)[K])
     {
-        return W.on && x >= W.x0 && y >= W.y0 && x + N <= W.x1 && y + N <= W.y1;
+        if (WPC == 1) return;
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) sh.red[phase][wave][k] = v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            int t = 0;
+#pragma unroll
+            for (int w = 0; w < WPC; ++w) t += sh.red[phase][w][k];
+            v[k] = __builtin_amdgcn_readfirstlane(t);
+        }
+        phase ^= 1;
     }
 
-    // Reference dword of unit u for the block at (x,y): from the LDS window
-    // when the block lies inside it, else from the reference plane.
-    __device__ __forceinline__ uint32_t ref4(int x, int y, int u, bool lds) const
+    __device__ __forceinline__ uint32_t ref4(int x, int y, int u) const
     {
-        if (lds) return ld4s((y - W.y0 + urow[u]) * W.pitch + (x - W.x0) + ucol[u]);
         return ld4g(rplane, static_cast<size_t>(y + urow[u]) * R.pitch + x + ucol[u]);
     }
 
     // SAD of the block at full-pel (x,y), which must fit in the frame.
-    __device__ __forceinline__ int32_t sad1(int x, int y) const
+    __device__ __forceinline__ int32_t sad1(int x, int y)
     {
-        const bool lds = in_win(x, y);
         if (Gm::G == 1) {
             uint32_t acc = 0;
 #pragma unroll
-            for (int u = 0; u < Gm::U; ++u) acc = __builtin_amdgcn_sad_u8(ref4(x, y, u, lds), o[u], acc);
-            return wave_sum(static_cast<int>(acc));
+            for (int u = 0; u < Gm::U; ++u) acc = __builtin_amdgcn_sad_u8(ref4(x, y, u), o[u], acc);
+            int32_t v[1] = {wave_sum(static_cast<int>(acc))};
+            block_sum<1>(v);
+            return v[0];
         }
-        const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ref4(x, y, 0, lds), o[0], 0u)));
+        const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ref4(x, y, 0), o[0], 0u)));
         return __builtin_amdgcn_readlane(v, 15);  // group 0 priced it; all groups did
     }
 
     // N >= 16: prices candidates first .. first+B-1 of pattern P (those < K)
-    // around (bx,by); all their loads are issued before the first SAD so the
-    // batch costs one memory latency.  Off-frame ones get kInf.
+    // around (bx,by), loads first; off-frame ones get kInf.
     template <int P>
     __device__ __forceinline__ void price_batch(int bx, int by, int scale, int first,
-                                                int32_t (&s)[Gm::B]) const
+                                                int32_t (&s)[Gm::B])
     {
         constexpr int K = Pat<P>::K;
         uint32_t v[Gm::B][Gm::U];
@@ -259,18 +286,20 @@ struct Searcher {
             ok[b] = k < K && fits(R, N, x, y);
             // a rejected candidate reads the (in-frame) centre instead
             const int lx = ok[b] ? x : bx, ly = ok[b] ? y : by;
-            const bool lds = in_win(lx, ly);
 #pragma unroll
-            for (int u = 0; u < Gm::U; ++u) v[b][u] = ref4(lx, ly, u, lds);
+            for (int u = 0; u < Gm::U; ++u) v[b][u] = ref4(lx, ly, u);
         }
 #pragma unroll
         for (int b = 0; b < Gm::B; ++b) {
             uint32_t acc = 0;
 #pragma unroll
             for (int u = 0; u < Gm::U; ++u) acc = __builtin_amdgcn_sad_u8(v[b][u], o[u], acc);
-            const int t = wave_sum(static_cast<int>(acc));
-            s[b] = ok[b] ? t : kInf;
+            s[b] = wave_sum(static_cast<int>(acc));
         }
+        block_sum<Gm::B>(s);
+#pragma unroll
+        for (int b = 0; b < Gm::B; ++b)
+            if (!ok[b]) s[b] = kInf;
     }
 
     // N == 8: prices candidates k >= first of a K-point set, four per DPP
@@ -299,7 +328,7 @@ struct Searcher {
                 }
             }
             if (!live) { x = 0; y = 0; }
-            const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ref4(x, y, 0, in_win(x, y)), o[0], 0u)));
+            const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ref4(x, y, 0), o[0], 0u)));
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int k = s0 + g;
@@ -384,70 +413,119 @@ struct Searcher {
 
     // ---- quarter-pel --------------------------------------------------------
 
-    // SAD of a quarter-pel candidate read directly from the planes (edge path).
-    __device__ __forceinline__ int32_t qsad_direct(int qx, int qy, int cls) const
+    // SAD of a quarter-pel candidate read directly from the planes (edge path),
+    // with the candidate's own class (1 six-tap, 2 bilinear).
+    __device__ __forceinline__ int32_t qsad_direct(int qx, int qy, int cls)
     {
         const int fx = qx >> 2, fy = qy >> 2, dx = qx & 3, dy = qy & 3;
-        const int lane = threadIdx.x & 63;
+        const anc_t *an = R.an;
         if (N == 8) {
             const int x = fx + (lane & 7), y = fy + (lane >> 3);
             int v;
-            if (cls == 1) {
-                const size_t e = static_cast<size_t>(y) * R.apitch + x;
-                v = qpix(aplane[e], aplane[e + 1], aplane[e + R.apitch], aplane[e + R.apitch + 1], dx, dy);
-            } else {
-                v = bil_px(R, x, y, dx, dy);
-            }
+            if (cls == 1) v = qpix_any(an + static_cast<size_t>(y) * R.apitch + x, R.apitch, dx, dy);
+            else v = bil_px(R, x, y, dx, dy);
             const int d = o8 - v;
             return wave_sum(d < 0 ? -d : d);
         }
+        // a rolled loop (this is the rare edge path): unit coordinates are
+        // recomputed and the current block re-read, so no register array is
+        // indexed dynamically
         uint32_t acc = 0;
 #pragma unroll 1
         for (int u = 0; u < Gm::U; ++u) {
-            const int unit = lane + 64 * u;
+            const int unit = wave * 64 + lane + 64 * WPC * u;
             const int ur = unit / Gm::DW, uc = (unit % Gm::DW) * 4;
             const int x = fx + uc, y = fy + ur;
             uint32_t pk = 0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 int v;
-                if (cls == 1) {
-                    const size_t e = static_cast<size_t>(y) * R.apitch + x + j;
-                    v = qpix(aplane[e], aplane[e + 1], aplane[e + R.apitch], aplane[e + R.apitch + 1], dx, dy);
-                } else {
-                    v = bil_px(R, x + j, y, dx, dy);
-                }
+                if (cls == 1) v = qpix_any(an + static_cast<size_t>(y) * R.apitch + x + j, R.apitch, dx, dy);
+                else v = bil_px(R, x + j, y, dx, dy);
                 pk |= static_cast<uint32_t>(v) << (8 * j);
             }
             acc = __builtin_amdgcn_sad_u8(pk, ld4g(cplane, cblk + static_cast<size_t>(ur) * cpitch + uc), acc);
         }
-        return wave_sum(static_cast<int>(acc));
+        int32_t v[1] = {wave_sum(static_cast<int>(acc))};
+        block_sum<1>(v);
+        return v[0];
     }
 
-    // Packed SAD of one unit (4 pixels) for a candidate whose anchor rows are
-    // ra (sample row) and rb (row below), starting at column ox.
-    template <int C>
-    __device__ __forceinline__ uint32_t unit_q(const anc_t (&ra)[C], const anc_t (&rb)[C], int ox,
-                                               int dx, int dy, uint32_t orig, uint32_t acc) const
+    __device__ static __forceinline__ int qpix_any(const anc_t *e, int stride, int dx, int dy)
     {
-        uint32_t pk = 0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const anc_t p00 = ox ? ra[j + 1] : ra[j], p10 = ox ? ra[j + 2] : ra[j + 1];
-            const anc_t p01 = ox ? rb[j + 1] : rb[j], p11 = ox ? rb[j + 2] : rb[j + 1];
-            pk |= static_cast<uint32_t>(qpix(p00, p10, p01, p11, dx, dy)) << (8 * j);
+        switch ((dy << 2) | dx) {
+        case 0: return qpix<0, 0>(e, stride);
+        case 1: return qpix<1, 0>(e, stride);
+        case 2: return qpix<2, 0>(e, stride);
+        case 3: return qpix<3, 0>(e, stride);
+        case 4: return qpix<0, 1>(e, stride);
+        case 5: return qpix<1, 1>(e, stride);
+        case 6: return qpix<2, 1>(e, stride);
+        case 7: return qpix<3, 1>(e, stride);
+        case 8: return qpix<0, 2>(e, stride);
+        case 9: return qpix<1, 2>(e, stride);
+        case 10: return qpix<2, 2>(e, stride);
+        case 11: return qpix<3, 2>(e, stride);
+        case 12: return qpix<0, 3>(e, stride);
+        case 13: return qpix<1, 3>(e, stride);
+        case 14: return qpix<2, 3>(e, stride);
+        default: return qpix<3, 3>(e, stride);
         }
-        return __builtin_amdgcn_sad_u8(pk, orig, acc);
     }
 
-    // Quarter-pel refinement, one greedy pass (reference src/motion.c:450-481).
-    // (bx,by) is the integer best.  When every sample position the pass can
-    // reach (fx in {bx-1, bx}, fy in {by-1, by}) is six-tap safe, the anchor
-    // neighbourhood is loaded once: N <= 16 keep it in registers and price
-    // candidates in the reference's order; N >= 32 price all remaining
-    // candidates per unit in rounds (first strict improvement wins, the rest
-    // are re-priced).  Otherwise candidates are priced one by one from the
-    // planes with their own class.
+    // SAD of a quarter-pel candidate from the LDS window: ox, oy (0 or 1)
+    // locate its sample origin in the window.
+    template <int DX, int DY>
+    __device__ __forceinline__ int32_t qsad_win(int ox, int oy)
+    {
+        const anc_t *w = sh.win + oy * Gm::WIN + ox;
+        if (N == 8) {
+            const int d = o8 - qpix<DX, DY>(w + (lane >> 3) * Gm::WIN + (lane & 7), Gm::WIN);
+            return wave_sum(d < 0 ? -d : d);
+        }
+        uint32_t acc = 0;
+#pragma unroll
+        for (int u = 0; u < Gm::U; ++u) {
+            const anc_t *e = w + urow[u] * Gm::WIN + ucol[u];
+            const uint32_t pk = static_cast<uint32_t>(qpix<DX, DY>(e, Gm::WIN)) |
+                                (static_cast<uint32_t>(qpix<DX, DY>(e + 1, Gm::WIN)) << 8) |
+                                (static_cast<uint32_t>(qpix<DX, DY>(e + 2, Gm::WIN)) << 16) |
+                                (static_cast<uint32_t>(qpix<DX, DY>(e + 3, Gm::WIN)) << 24);
+            acc = __builtin_amdgcn_sad_u8(pk, o[u], acc);
+        }
+        int32_t v[1] = {wave_sum(static_cast<int>(acc))};
+        block_sum<1>(v);
+        return v[0];
+    }
+
+    __device__ __forceinline__ int32_t qsad_win_any(int ox, int oy, int dx, int dy)
+    {
+        switch ((dy << 2) | dx) {
+        case 0: return qsad_win<0, 0>(ox, oy);
+        case 1: return qsad_win<1, 0>(ox, oy);
+        case 2: return qsad_win<2, 0>(ox, oy);
+        case 3: return qsad_win<3, 0>(ox, oy);
+        case 4: return qsad_win<0, 1>(ox, oy);
+        case 5: return qsad_win<1, 1>(ox, oy);
+        case 6: return qsad_win<2, 1>(ox, oy);
+        case 7: return qsad_win<3, 1>(ox, oy);
+        case 8: return qsad_win<0, 2>(ox, oy);
+        case 9: return qsad_win<1, 2>(ox, oy);
+        case 10: return qsad_win<2, 2>(ox, oy);
+        case 11: return qsad_win<3, 2>(ox, oy);
+        case 12: return qsad_win<0, 3>(ox, oy);
+        case 13: return qsad_win<1, 3>(ox, oy);
+        case 14: return qsad_win<2, 3>(ox, oy);
+        default: return qsad_win<3, 3>(ox, oy);
+        }
+    }
+
+    // Quarter-pel refinement, one greedy pass in the reference's order
+    // (reference src/motion.c:450-481).  (bx,by) is the integer best.  When
+    // every sample position the pass can reach (fx in {bx-1, bx}, fy in
+    // {by-1, by}) is six-tap safe, the block's (N+2)^2 anchor window is staged
+    // in LDS once and every candidate is six-tap; otherwise candidates are
+    // priced from the planes with their own class.
     __device__ __forceinline__ void qpel(int bx, int by, int &qx, int &qy, int32_t &best)
     {
         constexpr int K = Pat<PAT_QPEL>::K;
@@ -466,108 +544,28 @@ struct Searcher {
             }
             return;
         }
-        const int lane = threadIdx.x & 63;
         const int x0 = bx - 1, y0 = by - 1;  // window origin (sample coordinates)
-        if (N == 8) {
-            // lane = pixel; a 3x3 anchor window in registers
-            const int c = lane & 7, r = lane >> 3;
-            anc_t e[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    e[i][j] = aplane[static_cast<size_t>(y0 + r + i) * R.apitch + x0 + c + j];
-#pragma unroll 1
-            for (int k = 0; k < K; ++k) {
-                const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
-                const int ox = (tx >> 2) - x0, oy = (ty >> 2) - y0;
-                const int dx = tx & 3, dy = ty & 3;
-                cnt.q6++;
-                const anc_t p00 = oy ? (ox ? e[1][1] : e[1][0]) : (ox ? e[0][1] : e[0][0]);
-                const anc_t p10 = oy ? (ox ? e[1][2] : e[1][1]) : (ox ? e[0][2] : e[0][1]);
-                const anc_t p01 = oy ? (ox ? e[2][1] : e[2][0]) : (ox ? e[1][1] : e[1][0]);
-                const anc_t p11 = oy ? (ox ? e[2][2] : e[2][1]) : (ox ? e[1][2] : e[1][1]);
-                const int d = o8 - qpix(p00, p10, p01, p11, dx, dy);
-                const int32_t sv = wave_sum(d < 0 ? -d : d);
-                if (sv < best) { best = sv; qx = tx; qy = ty; }
-            }
-            return;
+        constexpr int WN = Gm::WIN * Gm::WIN;
+        for (int i = static_cast<int>(threadIdx.x); i < WN; i += 64 * WPC) {
+            const int r = i / Gm::WIN, c = i - r * Gm::WIN;
+            sh.win[i] = aplane[static_cast<size_t>(y0 + r) * R.apitch + x0 + c];
         }
-        if (Gm::U == 1) {
-            // N == 16: one unit per lane; a 3x6 anchor window in registers
-            anc_t e[3][6];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 6; ++j)
-                    e[i][j] = aplane[static_cast<size_t>(y0 + urow[0] + i) * R.apitch + x0 + ucol[0] + j];
+        __syncthreads();
 #pragma unroll 1
-            for (int k = 0; k < K; ++k) {
-                const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
-                const int ox = (tx >> 2) - x0, oy = (ty >> 2) - y0;
-                cnt.q6++;
-                uint32_t acc;
-                if (oy) acc = unit_q<6>(e[1], e[2], ox, tx & 3, ty & 3, o[0], 0u);
-                else acc = unit_q<6>(e[0], e[1], ox, tx & 3, ty & 3, o[0], 0u);
-                const int32_t sv = wave_sum(static_cast<int>(acc));
-                if (sv < best) { best = sv; qx = tx; qy = ty; }
-            }
-            return;
-        }
-        // N >= 32: rounds over the units, all remaining candidates per unit
-        int first = 0;
-#pragma unroll 1
-        while (first < K) {
-            uint32_t acc[K];
-#pragma unroll
-            for (int k = 0; k < K; ++k) acc[k] = 0;
-#pragma unroll 1
-            for (int u = 0; u < Gm::U; ++u) {
-                const int unit = lane + 64 * u;
-                const int ur = unit / Gm::DW, uc = (unit % Gm::DW) * 4;
-                anc_t e[3][6];
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int j = 0; j < 6; ++j)
-                        e[i][j] = aplane[static_cast<size_t>(y0 + ur + i) * R.apitch + x0 + uc + j];
-                const uint32_t orig = ld4g(cplane, cblk + static_cast<size_t>(ur) * cpitch + uc);
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    if (k < first) continue;
-                    const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
-                    const int ox = (tx >> 2) - x0, oy = (ty >> 2) - y0;
-                    if (oy) acc[k] = unit_q<6>(e[1], e[2], ox, tx & 3, ty & 3, orig, acc[k]);
-                    else acc[k] = unit_q<6>(e[0], e[1], ox, tx & 3, ty & 3, orig, acc[k]);
-                }
-            }
-            int hit = K, nx = qx, ny = qy;
-            int32_t nb = best;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (k >= first && hit == K) {
-                    cnt.q6++;
-                    const int32_t sv = wave_sum(static_cast<int>(acc[k]));
-                    if (sv < best) {
-                        hit = k;
-                        nb = sv;
-                        nx = qx + Pat<PAT_QPEL>::x(k);
-                        ny = qy + Pat<PAT_QPEL>::y(k);
-                    }
-                }
-            }
-            if (hit == K) break;
-            best = nb; qx = nx; qy = ny;
-            first = hit + 1;
+        for (int k = 0; k < K; ++k) {
+            const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
+            cnt.q6++;
+            const int32_t sv = qsad_win_any((tx >> 2) - x0, (ty >> 2) - y0, tx & 3, ty & 3);
+            if (sv < best) { best = sv; qx = tx; qy = ty; }
         }
     }
 };
 
-template <int N>
+template <int N, int WPC>
 __device__ void search(const PicDev &R, const PicDev &C, const tcme_call &c, tcme_result *res,
-                       tcme_stats *st, const Win &win)
+                       tcme_stats *st, Shared<N, WPC> &sh)
 {
-    Searcher<N> S(R, C, c.px, c.py, win);
+    Searcher<N, WPC> S(R, C, c.px, c.py, sh);
     // Centre clamp with tc_clip's order: low bound first (motion.c:391-392).
     int bx = c.cx < 0 ? 0 : (c.cx > R.w - N ? R.w - N : c.cx);
     int by = c.cy < 0 ? 0 : (c.cy > R.h - N ? R.h - N : c.cy);
@@ -588,7 +586,7 @@ __device__ void search(const PicDev &R, const PicDev &C, const tcme_call &c, tcm
         qx = bx * 4;
         qy = by * 4;
     }
-    if ((threadIdx.x & 63) == 0) {
+    if (threadIdx.x == 0) {
         res->mv_x = qx;
         res->mv_y = qy;
         res->sad = best;
@@ -616,47 +614,31 @@ __device__ __forceinline__ int check_call(const PicDev *table, int npics, const 
     return 0;
 }
 
-// Runs call i with the whole wave.
-__device__ __forceinline__ void run_call(const PicDev *table, int npics, const tcme_call &c, int i,
-                                         tcme_result *res, tcme_stats *stats, const Win &win)
+// Block index -> call index so that each of the 8 XCDs (which receive
+// workgroups round-robin) works through one contiguous run of the call list:
+// neighbouring calls read neighbouring reference rows, which then stay in
+// that XCD's L2.
+__device__ __forceinline__ int xcd_slot(int b, int nb)
 {
-    PicDev R{}, C{};
-    const int status = check_call(table, npics, c, R, C);
-    tcme_stats *st = stats ? stats + i : nullptr;
-    if (status) {
-        if ((threadIdx.x & 63) == 0) {
-            res[i].mv_x = 0;
-            res[i].mv_y = 0;
-            res[i].sad = kInf;
-            res[i].status = status;
-            if (st) { st->int_evals = st->q6_evals = st->qbil_evals = st->int_oob = 0; }
-        }
-        return;
-    }
-    switch (c.blk) {
-#ifndef TCME_ONLY_N
-    case 8: search<8>(R, C, c, res + i, st, win); break;
-    case 16: search<16>(R, C, c, res + i, st, win); break;
-    case 32: search<32>(R, C, c, res + i, st, win); break;
-    default: search<64>(R, C, c, res + i, st, win); break;
-#else
-    default: search<TCME_ONLY_N>(R, C, c, res + i, st, win); break;
-#endif
-    }
+    constexpr int kXcd = 8;
+    const int x = b % kXcd, slot = b / kXcd;
+    const int q = nb / kXcd, r = nb % kXcd;
+    return x * q + (x < r ? x : r) + slot;
 }
 
-// One wavefront per call, block size N only: the plain form launches one
-// instance per size (their register budgets differ 4x: an 8x8 search needs
-// about a quarter of a 64x64 one, so small blocks run at full occupancy).
-// A wave whose call has another size exits at once; the 64 instance also
-// answers descriptors with an unsupported size.
-template <int N>
-__global__ __launch_bounds__(64) void me_search_kernel(const PicDev *__restrict__ table, int npics,
-                                                        const tcme_call *__restrict__ calls, int n,
-                                                        tcme_result *__restrict__ res,
-                                                        tcme_stats *__restrict__ stats)
+// One workgroup of WPC wavefronts per call, block size N only: each size has
+// its own instance (the register and LDS budgets differ ~16x between 8x8 and
+// 64x64).  A workgroup whose call has another size exits at once; the 64
+// instance also answers descriptors with an unsupported size.
+template <int N, int WPC>
+__global__ __launch_bounds__(64 * WPC) void me_search_kernel(const PicDev *__restrict__ table,
+                                                              int npics,
+                                                              const tcme_call *__restrict__ calls,
+                                                              int n, tcme_result *__restrict__ res,
+                                                              tcme_stats *__restrict__ stats)
 {
-    const int i = blockIdx.x;
+    __shared__ Shared<N, WPC> sh;
+    const int i = xcd_slot(blockIdx.x, gridDim.x);
     if (i >= n) return;
     const tcme_call c = calls[i];
     const bool mine = c.blk == N || (N == 64 && c.blk != 8 && c.blk != 16 && c.blk != 32);
@@ -674,105 +656,7 @@ __global__ __launch_bounds__(64) void me_search_kernel(const PicDev *__restrict_
         }
         return;
     }
-    const Win none{0, 0, 0, 0, 0, false};
-    search<N>(R, C, c, res + i, st, none);
-}
-
-// Block index -> chunk index so that each of the 8 XCDs (which receive
-// workgroups round-robin) works on one contiguous run of chunks: neighbouring
-// chunks read neighbouring reference rows, which then stay in that XCD's L2.
-__device__ __forceinline__ int xcd_chunk(int b, int nb)
-{
-    constexpr int kXcd = 8;
-    const int x = b % kXcd, slot = b / kXcd;
-    const int q = nb / kXcd, r = nb % kXcd;
-    return x * q + (x < r ? x : r) + slot;
-}
-
-// Relative cost of one call, for splitting a chunk between wavefronts.
-__device__ __forceinline__ int call_cost(int blk)
-{
-    return blk * blk + 256;
-}
-
-// Chunked search: a workgroup of WAVES wavefronts takes `chunk` consecutive
-// calls and stages the union of their reference windows (clamped centre +-
-// apron, one reference picture) in LDS with coalesced dword loads.  Each
-// wavefront then takes a contiguous, cost-balanced share of the chunk and
-// prices integer candidates from LDS; a candidate outside the window reads
-// the reference plane instead, so the apron only affects speed, never
-// results.  All control flow below the barriers is wave-uniform: the
-// wavefront index comes from readfirstlane and the share from scalar loads of
-// the call list.
-template <int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(
-    const PicDev *__restrict__ table, int npics, const tcme_call *__restrict__ calls, int n,
-    int chunk, int apron, int lds_cap, tcme_result *__restrict__ res,
-    tcme_stats *__restrict__ stats)
-{
-    __shared__ int box[5];
-    const int tid = threadIdx.x;
-    const int b = xcd_chunk(blockIdx.x, gridDim.x);
-    const int c0 = b * chunk;
-    const int c1 = min(n, c0 + chunk);
-    if (c0 >= n) return;  // uniform for the whole workgroup
-    if (tid == 0) {
-        box[0] = 0x7FFFFFFF; box[1] = 0x7FFFFFFF;
-        box[2] = -0x7FFFFFFF; box[3] = -0x7FFFFFFF;
-        box[4] = calls[c0].ref_pic;
-    }
-    __syncthreads();
-    const int ref0 = box[4];
-    for (int i = c0 + tid; i < c1; i += 64 * WAVES) {
-        const tcme_call c = calls[i];
-        PicDev R{}, C{};
-        if (c.ref_pic != ref0 || check_call(table, npics, c, R, C)) continue;
-        const int n_ = c.blk;
-        const int cx = c.cx < 0 ? 0 : (c.cx > R.w - n_ ? R.w - n_ : c.cx);
-        const int cy = c.cy < 0 ? 0 : (c.cy > R.h - n_ ? R.h - n_ : c.cy);
-        atomicMin(&box[0], cx - apron);
-        atomicMin(&box[1], cy - apron);
-        atomicMax(&box[2], cx + n_ + apron);
-        atomicMax(&box[3], cy + n_ + apron);
-    }
-    __syncthreads();
-    Win win{0, 0, 0, 0, 0, false};
-    if (box[0] != 0x7FFFFFFF) {
-        const PicDev R = table[ref0];
-        const int x0 = (box[0] < 0 ? 0 : box[0]) & ~3;
-        const int y0 = box[1] < 0 ? 0 : box[1];
-        const int x1 = box[2] > R.w ? R.w : box[2];
-        const int y1 = box[3] > R.h ? R.h : box[3];
-        const int pitch = ((x1 - x0 + 3) & ~3) + 4;
-        const int rows = y1 - y0;
-        if (x1 > x0 && rows > 0 && rows * pitch + 8 <= lds_cap) {
-            const int words = pitch / 4;
-            gptr32 src = as_global32(R.y);
-            for (int i = tid; i < rows * words; i += 64 * WAVES) {
-                const int r = i / words, cw = i - r * words;
-                tcme_win[i] = src[(static_cast<size_t>(y0 + r) * R.pitch + x0) / 4 + cw];
-            }
-            if (tid == 0) tcme_win[rows * words] = 0;  // the pair read past the last row
-            win = Win{x0, y0, x1, y1, pitch, true};
-        }
-    }
-    __syncthreads();
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    int total = 0;
-    for (int j = c0; j < c1; ++j) total += call_cost(calls[j].blk);
-    const int lo = static_cast<int>(static_cast<long long>(total) * wave / WAVES);
-    const int hi = static_cast<int>(static_cast<long long>(total) * (wave + 1) / WAVES);
-    int acc = 0;
-    for (int j = c0; j < c1; ++j) {
-        const tcme_call c = calls[j];
-        const int cj = call_cost(c.blk);
-        const int mid = acc + cj / 2;
-        acc += cj;
-        if (mid < lo || mid >= hi) continue;
-        Win w = win;
-        w.on = win.on && c.ref_pic == ref0;
-        run_call(table, npics, c, j, res, stats, w);
-    }
+    search<N, WPC>(R, C, c, res + i, st, sh);
 }
 
 // Anchor plane of a picture (reference src/motion.c:70-107 for the taps):
@@ -818,52 +702,47 @@ hipError_t launch_anchors(const PicDev &p, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_search_n(int blk, const PicDev *table, int npics, const tcme_call *calls, int n,
-                           tcme_result *res, tcme_stats *stats, hipStream_t s)
+namespace {
+
+template <int N, int WPC>
+hipError_t launch_one(const PicDev *table, int npics, const tcme_call *calls, int n,
+                      tcme_result *res, tcme_stats *stats, hipStream_t s)
 {
-    if (n <= 0) return hipSuccess;
-    switch (blk) {
-    case 8:
-        hipLaunchKernelGGL(me_search_kernel<8>, dim3(n), dim3(64), 0, s, table, npics, calls, n, res, stats);
-        break;
-    case 16:
-        hipLaunchKernelGGL(me_search_kernel<16>, dim3(n), dim3(64), 0, s, table, npics, calls, n, res, stats);
-        break;
-    case 32:
-        hipLaunchKernelGGL(me_search_kernel<32>, dim3(n), dim3(64), 0, s, table, npics, calls, n, res, stats);
-        break;
-    default:
-        hipLaunchKernelGGL(me_search_kernel<64>, dim3(n), dim3(64), 0, s, table, npics, calls, n, res, stats);
-        break;
-    }
+    hipLaunchKernelGGL((me_search_kernel<N, WPC>), dim3(n), dim3(64 * WPC), 0, s, table, npics,
+                       calls, n, res, stats);
     return hipGetLastError();
 }
 
-hipError_t launch_search_tiled(const PicDev *table, int npics, const tcme_call *calls, int n,
-                               const Tiling &t, tcme_result *res, tcme_stats *stats,
-                               hipStream_t s)
+}  // namespace
+
+bool valid_waves(int blk, int waves)
+{
+    switch (blk) {
+    case 8: return waves == 1;
+    case 16: return waves == 1;
+    case 32: return waves == 1 || waves == 2 || waves == 4;
+    case 64: return waves == 2 || waves == 4 || waves == 8;
+    default: return false;
+    }
+}
+
+hipError_t launch_search_n(int blk, int waves, const PicDev *table, int npics,
+                           const tcme_call *calls, int n, tcme_result *res, tcme_stats *stats,
+                           hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    const int nb = (n + t.chunk - 1) / t.chunk;
-    switch (t.waves) {
-    case 1:
-        hipLaunchKernelGGL(me_tile_kernel<1>, dim3(nb), dim3(64), t.lds_bytes, s, table, npics,
-                           calls, n, t.chunk, t.apron, t.lds_bytes, res, stats);
-        break;
-    case 2:
-        hipLaunchKernelGGL(me_tile_kernel<2>, dim3(nb), dim3(128), t.lds_bytes, s, table, npics,
-                           calls, n, t.chunk, t.apron, t.lds_bytes, res, stats);
-        break;
-    case 8:
-        hipLaunchKernelGGL(me_tile_kernel<8>, dim3(nb), dim3(512), t.lds_bytes, s, table, npics,
-                           calls, n, t.chunk, t.apron, t.lds_bytes, res, stats);
-        break;
+    switch (blk) {
+    case 8: return launch_one<8, 1>(table, npics, calls, n, res, stats, s);
+    case 16: return launch_one<16, 1>(table, npics, calls, n, res, stats, s);
+    case 32:
+        if (waves == 1) return launch_one<32, 1>(table, npics, calls, n, res, stats, s);
+        if (waves == 4) return launch_one<32, 4>(table, npics, calls, n, res, stats, s);
+        return launch_one<32, 2>(table, npics, calls, n, res, stats, s);
     default:
-        hipLaunchKernelGGL(me_tile_kernel<4>, dim3(nb), dim3(256), t.lds_bytes, s, table, npics,
-                           calls, n, t.chunk, t.apron, t.lds_bytes, res, stats);
-        break;
+        if (waves == 2) return launch_one<64, 2>(table, npics, calls, n, res, stats, s);
+        if (waves == 8) return launch_one<64, 8>(table, npics, calls, n, res, stats, s);
+        return launch_one<64, 4>(table, npics, calls, n, res, stats, s);
     }
-    return hipGetLastError();
 }
 
 }  // namespace tcme

@@ -45,8 +45,9 @@ class MV(C.Structure):
 
 
 class Tiling(C.Structure):
-    _fields_ = [("chunk", C.c_int32), ("waves", C.c_int32), ("apron", C.c_int32),
-                ("lds_bytes", C.c_int32)]
+    """Wavefronts per call for each block size."""
+    _fields_ = [("waves8", C.c_int32), ("waves16", C.c_int32), ("waves32", C.c_int32),
+                ("waves64", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -205,8 +206,8 @@ class Engine:
         _check(self.L.tcme_get_tiling(self.h, C.byref(t)))
         return t
 
-    def set_tiling(self, chunk: int, waves: int, apron: int, lds_bytes: int) -> None:
-        t = Tiling(chunk, waves, apron, lds_bytes)
+    def set_tiling(self, waves8: int, waves16: int, waves32: int, waves64: int) -> None:
+        t = Tiling(waves8, waves16, waves32, waves64)
         _check(self.L.tcme_set_tiling(self.h, C.byref(t)))
 
     def autotune(self, calls_ptr: int, n: int, res_ptr: int, reps: int = 3, stream: int = 0):
