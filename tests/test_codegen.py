@@ -1,0 +1,42 @@
+"""Checks on the gfx950 assembly of the search kernels (built here, no GPU).
+
+LDS must only be accessed with naturally aligned instructions: an 8-byte
+anchor read narrowed by the compiler to a 4-byte read at a 2-byte offset
+returns the wrong bytes on the device.  LDS must also never be reached
+through flat (generic) instructions built from integer-cast pointers."""
+import os
+import re
+import subprocess
+
+from conftest import ROOT
+
+CSRC = os.path.join(ROOT, "tcodec_amd", "csrc")
+ASM = os.path.join(ROOT, "tcodec_amd", "lib", "tcme_kernels.s")
+SIZE = {"b32": 4, "b64": 8, "b96": 16, "b128": 16, "u16": 2, "i16": 2, "b16": 2, "u8": 1,
+        "i8": 1, "b8": 1}
+
+
+def assembly() -> str:
+    subprocess.run(["make", "-s", "-C", CSRC, "asm"], check=True)
+    return open(ASM).read()
+
+
+def test_lds_accesses_are_naturally_aligned():
+    text = assembly()
+    bad = []
+    for m in re.finditer(r"\n\s+ds_(read|write)(2?)_(\w+?) [^\n]*?((?:offset\d?:\d+\s*)*)\n", text):
+        kind, two, ty, offs = m.groups()
+        size = SIZE.get(ty)
+        if size is None or two:
+            continue  # read2/write2 offsets count elements, not bytes
+        for off in re.findall(r"offset:(\d+)", offs):
+            if int(off) % size:
+                bad.append(m.group(0).strip())
+    assert not bad, bad[:5]
+
+
+def test_every_kernel_present():
+    text = assembly()
+    for n, wpc in [(8, 1), (16, 1), (32, 1), (32, 2), (32, 4), (64, 2), (64, 4), (64, 8)]:
+        assert re.search(rf"me_search_kernelILi{n}ELi{wpc}E", text), (n, wpc)
+    assert "anchor_kernel" in text
