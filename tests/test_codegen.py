@@ -1,9 +1,9 @@
 """Checks on the gfx950 assembly of the search kernels (built here, no GPU).
 
-LDS must only be accessed with naturally aligned instructions: an 8-byte
-anchor read narrowed by the compiler to a 4-byte read at a 2-byte offset
-returns the wrong bytes on the device.  LDS must also never be reached
-through flat (generic) instructions built from integer-cast pointers."""
+v_ashr_pk_u8_i32 writes only the low half of its destination on the device,
+while the compiler ORs further bytes into the upper half as if it were zero;
+the library is built with that instruction disabled and must not contain it.
+LDS accesses stay naturally aligned (anchors are read as whole 8-byte words)."""
 import os
 import re
 import subprocess
@@ -33,6 +33,11 @@ def test_lds_accesses_are_naturally_aligned():
             if int(off) % size:
                 bad.append(m.group(0).strip())
     assert not bad, bad[:5]
+
+
+def test_no_packed_shift_clip():
+    text = assembly()
+    assert "v_ashr_pk_" not in text
 
 
 def test_every_kernel_present():
