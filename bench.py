@@ -52,8 +52,10 @@ def build_workload(eng):
 
 
 def kernel_name(tiling: dict) -> str:
-    return ("me_search_kernel<8,1>+<16,1>+<32,%d>+<64,%d> (concurrent)"
-            % (tiling["waves32"], tiling["waves64"]))
+    small = ("me_tile_kernel<%d>" % tiling["tile_waves"] if tiling["tile_waves"]
+             else "me_search_kernel<8,1>+<16,1>")
+    return ("%s+me_search_kernel<32,%d>+<64,%d> (concurrent)"
+            % (small, tiling["waves32"], tiling["waves64"]))
 
 
 def measured_traffic(tiling: dict):
@@ -115,8 +117,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tiling", default=None,
-                    help="w8,w16,w32,w64 wavefronts per call by block size: use this launch "
-                         "shape instead of autotuning")
+                    help="w8,w16,w32,w64[,tile_waves,tile_chunk]: wavefronts per call by "
+                         "block size and the tile workgroups for 8x8/16x16 calls; use this "
+                         "launch shape instead of autotuning")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (counter calibration)")
     ap.add_argument("--mode", choices=("tuned", "default"), default="tuned",

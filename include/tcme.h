@@ -100,11 +100,19 @@ int  tcme_pic_finish(tcme_ctx *ctx, int pic, void *stream);
 int  tcme_search(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,
                  tcme_stats *stats, void *stream);
 
-/* Launch shape of tcme_search: wavefronts cooperating on one call, per
- * block size (8x8: 1; 16x16: 1; 32x32: 1, 2 or 4; 64x64: 2, 4 or 8).
- * Results never depend on it. */
+/* Launch shape of tcme_search.  Results never depend on it.
+ *   waves8..waves64  wavefronts cooperating on one call, per block size
+ *                    (8x8: 1; 16x16: 1; 32x32: 1, 2 or 4; 64x64: 2, 4 or 8)
+ *   tile_waves       0: every 8x8 and 16x16 call gets its own workgroup;
+ *                    4 or 8: tile workgroups of that many wavefronts each take
+ *                    tile_chunk (1..128) consecutive calls, stage the reference
+ *                    around each run of calls that share a reference picture
+ *                    and a 64x64 centre tile in LDS, and search the run's 8x8
+ *                    and 16x16 calls from there (lists grouped by CTU and
+ *                    reference, 85 calls per group, suit tile_chunk 85) */
 typedef struct tcme_tiling {
     int32_t waves8, waves16, waves32, waves64;
+    int32_t tile_waves, tile_chunk;
 } tcme_tiling;
 
 int  tcme_set_tiling(tcme_ctx *ctx, const tcme_tiling *t);

@@ -151,9 +151,10 @@ int pic_free(tcme_ctx *c, int id)
     return TCME_OK;
 }
 
-// Launches the search of n calls in launch shape t on stream s: the four
-// per-size kernels run concurrently on the context's streams, forked from and
-// joined back into s with events.
+// Launches the search of n calls in launch shape t on stream s: the kernels
+// (per-size for 64x64 and 32x32; the tile kernel or per-size ones for 16x16
+// and 8x8) run concurrently on the context's streams, forked from and joined
+// back into s with events.
 int dispatch(tcme_ctx *c, const tcme_call *calls, int n, const tcme::Tiling &t,
              tcme_result *res, tcme_stats *stats, hipStream_t s)
 {
@@ -161,11 +162,16 @@ int dispatch(tcme_ctx *c, const tcme_call *calls, int n, const tcme::Tiling &t,
     std::lock_guard<std::mutex> g(c->launch_mu);
     static const int sizes[4] = {64, 32, 16, 8};
     static const int slot[4] = {3, 2, 1, 0};
+    const int nk = t.tile_waves ? 3 : 4;
     HIP_TRY(hipEventRecord(c->fork, s));
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < nk; ++k) {
         HIP_TRY(hipStreamWaitEvent(c->sub[k], c->fork, 0));
-        HIP_TRY(tcme::launch_search_n(sizes[k], t.waves[slot[k]], c->d_table, c->max_pics, calls,
+        if (t.tile_waves && k == 2)
+            HIP_TRY(tcme::launch_tile(t.tile_waves, t.tile_chunk, c->d_table, c->max_pics, calls,
                                       n, res, stats, c->sub[k]));
+        else
+            HIP_TRY(tcme::launch_search_n(sizes[k], t.waves[slot[k]], c->d_table, c->max_pics,
+                                          calls, n, res, stats, c->sub[k]));
         HIP_TRY(hipEventRecord(c->join[k], c->sub[k]));
         HIP_TRY(hipStreamWaitEvent(s, c->join[k], 0));
     }
@@ -312,6 +318,10 @@ int check_tiling(const tcme_tiling *t)
     for (int k = 0; k < 4; ++k)
         if (!tcme::valid_waves(8 << k, w[k]))
             return fail(TCME_E_ARG, "%d waves per %dx%d call not supported", w[k], 8 << k, 8 << k);
+    if (t->tile_waves != 0 && t->tile_waves != 4 && t->tile_waves != 8)
+        return fail(TCME_E_ARG, "%d waves per tile workgroup not supported", t->tile_waves);
+    if (t->tile_waves && (t->tile_chunk < 1 || t->tile_chunk > tcme::kTileChunkMax))
+        return fail(TCME_E_ARG, "tile chunk %d outside 1..%d", t->tile_chunk, tcme::kTileChunkMax);
     return TCME_OK;
 }
 
@@ -322,6 +332,8 @@ tcme::Tiling to_internal(const tcme_tiling &t)
     r.waves[1] = t.waves16;
     r.waves[2] = t.waves32;
     r.waves[3] = t.waves64;
+    r.tile_waves = t.tile_waves;
+    r.tile_chunk = t.tile_waves ? t.tile_chunk : r.tile_chunk;
     return r;
 }
 
@@ -342,13 +354,16 @@ TCME_API int tcme_get_tiling(tcme_ctx *c, tcme_tiling *t)
     t->waves16 = c->tiling.waves[1];
     t->waves32 = c->tiling.waves[2];
     t->waves64 = c->tiling.waves[3];
+    t->tile_waves = c->tiling.tile_waves;
+    t->tile_chunk = c->tiling.tile_chunk;
     return TCME_OK;
 }
 
-// Times every launch shape (wavefronts per 32x32 and per 64x64 call) on the
-// caller's call list (device arrays; results are overwritten with the same
-// answers) and keeps the fastest in the context.  Results do not depend on
-// the shape, only time does.
+// Times every launch shape (wavefronts per 32x32 and per 64x64 call, tile
+// workgroups of 0, 4 or 8 wavefronts; the tile chunk stays the context's) on
+// the caller's call list (device arrays; results are overwritten with the
+// same answers) and keeps the fastest in the context.  Results do not depend
+// on the shape, only time does.
 TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_result *results,
                            int reps, void *stream, tcme_tiling *best_out, float *best_ms)
 {
@@ -359,27 +374,32 @@ TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_resu
     hipStream_t s = static_cast<hipStream_t>(stream);
     static const int w32[] = {1, 2, 4};
     static const int w64[] = {2, 4, 8};
+    static const int tw[] = {0, 4, 8};
+    const int chunk = c->tiling.tile_chunk;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     tcme::Tiling best;
     float best_t = 1e30f;
     for (int a : w32)
-        for (int b : w64) {
-            tcme::Tiling t;
-            t.waves[2] = a;
-            t.waves[3] = b;
-            if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;  // warm
-            HIP_TRY(hipEventRecord(e0, s));
-            for (int r = 0; r < reps; ++r)
-                if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;
-            HIP_TRY(hipEventRecord(e1, s));
-            HIP_TRY(hipEventSynchronize(e1));
-            float ms = 0;
-            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            ms /= reps;
-            if (ms < best_t) { best_t = ms; best = t; }
-        }
+        for (int b : w64)
+            for (int tl : tw) {
+                tcme::Tiling t;
+                t.waves[2] = a;
+                t.waves[3] = b;
+                t.tile_waves = tl;
+                t.tile_chunk = chunk;
+                if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;  // warm
+                HIP_TRY(hipEventRecord(e0, s));
+                for (int r = 0; r < reps; ++r)
+                    if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;
+                HIP_TRY(hipEventRecord(e1, s));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                ms /= reps;
+                if (ms < best_t) { best_t = ms; best = t; }
+            }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->tiling = best;

@@ -7,6 +7,9 @@
 //                     (reference src/motion.c:384-485): one workgroup per call,
 //                     WPC wavefronts splitting its block, one instance per
 //                     block size
+//   me_tile_kernel    the same search for the 8x8 and 16x16 calls, one
+//                     wavefront per call, many calls per workgroup sharing a
+//                     reference window staged in LDS
 //
 // The walk is the reference's sequential greedy walk.  Its decisions are
 // wave-uniform (scalar unit); lanes price candidates.  Candidates may be
@@ -36,9 +39,11 @@ __device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f)
 
 // Global-memory views (address space 1), so plane reads compile to global
 // loads rather than flat ones.
+typedef const __attribute__((address_space(1))) uint8_t *gptr8;
 typedef const __attribute__((address_space(1))) uint32_t *gptr32;
 typedef const __attribute__((address_space(1))) uint64_t *gptr64;
 
+__device__ __forceinline__ gptr8 as_global8(const void *p) { return (gptr8)(p); }
 __device__ __forceinline__ gptr32 as_global32(const void *p) { return (gptr32)(p); }
 __device__ __forceinline__ gptr64 as_global64(const void *p) { return (gptr64)(p); }
 
@@ -115,7 +120,7 @@ __device__ __forceinline__ int a_c(anc_t e) { return static_cast<int>(static_cas
 // (reference src/motion.c:111-116, 330-336); never leaves [0,255].
 __device__ __forceinline__ int bil_px(const PicDev &R, int x, int y, int dx, int dy)
 {
-    const uint8_t *p = R.y + static_cast<size_t>(y) * R.pitch + x;
+    const gptr8 p = as_global8(R.y) + static_cast<size_t>(y) * R.pitch + x;
     const int top = (p[0] * (4 - dx) + p[1] * dx + 2) >> 2;
     const int bot = (p[R.pitch] * (4 - dx) + p[R.pitch + 1] * dx + 2) >> 2;
     return (top * (4 - dy) + bot * dy + 2) >> 2;
@@ -146,8 +151,9 @@ __device__ __forceinline__ int lat(int l0, int l1, int l2)
     return (l1 + l2 + 1) >> 1;
 }
 
-template <int DX, int DY>
-__device__ __forceinline__ int qpix(const anc_t *e, int stride)
+// (E: an LDS anchor pointer or a global one, gptr64.)
+template <int DX, int DY, typename E>
+__device__ __forceinline__ int qpix(E e, int stride)
 {
     // e points at the sample's entry; e[1] is right of it, e[stride] below
     const anc_t p00 = e[0];
@@ -188,17 +194,36 @@ struct Shared {
     int red[2][WPC][8];            // cross-wavefront partial sums (double-buffered)
 };
 
-template <int N, int WPC>
+// Reference luma window staged in LDS by a tile workgroup: samples
+// [x0,x1) x [y0,y1) of the reference, row r at byte r * kWinPitch.
+constexpr int kWinMargin = 40;                  // samples around the 64x64 centre tile
+constexpr int kWinSide = 64 + 2 * kWinMargin;   // 144
+constexpr int kWinPitch = kWinSide + 4;         // 148 bytes = 37 dwords per row (odd: banks)
+constexpr int kWinWords = kWinSide * kWinPitch / 4;
+
+struct LumaWin {
+    const uint32_t *w;  // LDS
+    int x0, y0, x1, y1;
+};
+
+// Full-pel candidate search + quarter-pel refinement of one call by WPC
+// wavefronts.  qw ((N+2)^2 anchors) and red ([2][WPC][8] ints) are LDS areas
+// owned by this call's wavefronts.  With WIN, integer candidates that lie
+// inside the LDS luma window are read from it instead of the plane.
+template <int N, int WPC, bool WIN = false>
 struct Searcher {
     using Gm = Geo<N, WPC>;
     const PicDev &R;
-    Shared<N, WPC> &sh;
+    anc_t *qw;
+    int *red;
+    LumaWin lw;
     uint32_t o[Gm::U];    // current block, packed bytes per unit
     int urow[Gm::U], ucol[Gm::U];
     int lane, wave;
+    int tid;              // rank among the call's 64 * WPC threads
     int grp;              // candidate group (N == 8)
     int o8;               // N == 8: this lane's current-block pixel
-    int phase;            // which half of sh.red the next exchange uses
+    int phase;            // which half of red the next exchange uses
     gptr32 cplane;        // current picture
     size_t cblk;          // byte offset of the current block in it
     int cpitch;
@@ -206,13 +231,15 @@ struct Searcher {
     gptr64 aplane;        // reference anchors
     Counts cnt;
 
-    __device__ Searcher(const PicDev &ref, const PicDev &cur, int px, int py, Shared<N, WPC> &s)
-        : R(ref), sh(s), phase(0), cplane(as_global32(cur.y)),
+    __device__ Searcher(const PicDev &ref, const PicDev &cur, int px, int py, anc_t *qwin,
+                        int *redbuf, LumaWin win = LumaWin{nullptr, 0, 0, 0, 0})
+        : R(ref), qw(qwin), red(redbuf), lw(win), phase(0), cplane(as_global32(cur.y)),
           cblk(static_cast<size_t>(py) * cur.pitch + px), cpitch(cur.pitch),
           rplane(as_global32(ref.y)), aplane(as_global64(ref.an))
     {
         lane = threadIdx.x & 63;
-        wave = WPC == 1 ? 0 : __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+        tid = static_cast<int>(threadIdx.x) & (64 * WPC - 1);
+        wave = WPC == 1 ? 0 : __builtin_amdgcn_readfirstlane(tid >> 6);
         grp = Gm::G == 1 ? 0 : lane / Gm::UNITS;
 #pragma unroll
         for (int u = 0; u < Gm::U; ++u) {
@@ -237,21 +264,49 @@ struct Searcher {
         if (WPC == 1) return;
         if (lane == 0) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) sh.red[phase][wave][k] = v[k];
+            for (int k = 0; k < K; ++k) red[(phase * WPC + wave) * 8 + k] = v[k];
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             int t = 0;
 #pragma unroll
-            for (int w = 0; w < WPC; ++w) t += sh.red[phase][w][k];
+            for (int w = 0; w < WPC; ++w) t += red[(phase * WPC + w) * 8 + k];
             v[k] = __builtin_amdgcn_readfirstlane(t);
         }
         phase ^= 1;
     }
 
-    __device__ __forceinline__ uint32_t ref4(int x, int y, int u) const
+    // Whether the N x N block at (x,y) lies inside the LDS luma window.
+    __device__ __forceinline__ bool inwin(int x, int y) const
     {
+        return WIN && x >= lw.x0 && y >= lw.y0 && x + N <= lw.x1 && y + N <= lw.y1;
+    }
+
+    // Four reference samples of unit u of the block at (x,y): from the LDS
+    // window when `w` (the block lies inside it), else from the plane.
+    // Makes this call's LDS writes visible to all of its threads.  A call of
+    // one wavefront syncs only that wavefront (a tile workgroup's wavefronts
+    // work on different calls); LDS operations of one wavefront complete in
+    // order, so ordering the compiler is enough.
+    __device__ __forceinline__ void sync_call() const
+    {
+        if (WPC == 1) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
+    }
+
+    __device__ __forceinline__ uint32_t ref4(int x, int y, int u, bool w = false) const
+    {
+        if (WIN && w) {
+            const int a = (y - lw.y0 + urow[u]) * kWinPitch + (x - lw.x0) + ucol[u];
+            const uint32_t *q = lw.w + (a >> 2);
+            return __builtin_amdgcn_alignbyte(q[1], q[0], static_cast<uint32_t>(a & 3));
+        }
         return ld4g(rplane, static_cast<size_t>(y + urow[u]) * R.pitch + x + ucol[u]);
     }
 
@@ -260,13 +315,14 @@ struct Searcher {
     {
         if (Gm::G == 1) {
             uint32_t acc = 0;
+            const bool w = inwin(x, y);
 #pragma unroll
-            for (int u = 0; u < Gm::U; ++u) acc = __builtin_amdgcn_sad_u8(ref4(x, y, u), o[u], acc);
+            for (int u = 0; u < Gm::U; ++u) acc = __builtin_amdgcn_sad_u8(ref4(x, y, u, w), o[u], acc);
             int32_t v[1] = {wave_sum(static_cast<int>(acc))};
             block_sum<1>(v);
             return v[0];
         }
-        const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ref4(x, y, 0), o[0], 0u)));
+        const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ref4(x, y, 0, inwin(x, y)), o[0], 0u)));
         return __builtin_amdgcn_readlane(v, 15);  // group 0 priced it; all groups did
     }
 
@@ -286,8 +342,9 @@ struct Searcher {
             ok[b] = k < K && fits(R, N, x, y);
             // a rejected candidate reads the (in-frame) centre instead
             const int lx = ok[b] ? x : bx, ly = ok[b] ? y : by;
+            const bool w = inwin(lx, ly);
 #pragma unroll
-            for (int u = 0; u < Gm::U; ++u) v[b][u] = ref4(lx, ly, u);
+            for (int u = 0; u < Gm::U; ++u) v[b][u] = ref4(lx, ly, u, w);
         }
 #pragma unroll
         for (int b = 0; b < Gm::B; ++b) {
@@ -328,7 +385,10 @@ struct Searcher {
                 }
             }
             if (!live) { x = 0; y = 0; }
-            const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ref4(x, y, 0), o[0], 0u)));
+            // one source for the whole wavefront: the window only when all
+            // four groups' candidates lie inside it
+            const bool w = WIN && __ballot(!inwin(x, y)) == 0;
+            const int v = row_scan(static_cast<int>(__builtin_amdgcn_sad_u8(ref4(x, y, 0, w), o[0], 0u)));
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int k = s0 + g;
@@ -418,7 +478,7 @@ struct Searcher {
     __device__ __forceinline__ int32_t qsad_direct(int qx, int qy, int cls)
     {
         const int fx = qx >> 2, fy = qy >> 2, dx = qx & 3, dy = qy & 3;
-        const anc_t *an = R.an;
+        const gptr64 an = aplane;
         if (N == 8) {
             const int x = fx + (lane & 7), y = fy + (lane >> 3);
             int v;
@@ -451,7 +511,8 @@ struct Searcher {
         return v[0];
     }
 
-    __device__ static __forceinline__ int qpix_any(const anc_t *e, int stride, int dx, int dy)
+    template <typename E>
+    __device__ static __forceinline__ int qpix_any(E e, int stride, int dx, int dy)
     {
         switch ((dy << 2) | dx) {
         case 0: return qpix<0, 0>(e, stride);
@@ -478,7 +539,7 @@ struct Searcher {
     template <int DX, int DY>
     __device__ __forceinline__ int32_t qsad_win(int ox, int oy)
     {
-        const anc_t *w = sh.win + oy * Gm::WIN + ox;
+        const anc_t *w = qw + oy * Gm::WIN + ox;
         if (N == 8) {
             const int d = o8 - qpix<DX, DY>(w + (lane >> 3) * Gm::WIN + (lane & 7), Gm::WIN);
             return wave_sum(d < 0 ? -d : d);
@@ -546,11 +607,11 @@ struct Searcher {
         }
         const int x0 = bx - 1, y0 = by - 1;  // window origin (sample coordinates)
         constexpr int WN = Gm::WIN * Gm::WIN;
-        for (int i = static_cast<int>(threadIdx.x); i < WN; i += 64 * WPC) {
+        for (int i = tid; i < WN; i += 64 * WPC) {
             const int r = i / Gm::WIN, c = i - r * Gm::WIN;
-            sh.win[i] = aplane[static_cast<size_t>(y0 + r) * R.apitch + x0 + c];
+            qw[i] = aplane[static_cast<size_t>(y0 + r) * R.apitch + x0 + c];
         }
-        __syncthreads();
+        sync_call();
 #pragma unroll 1
         for (int k = 0; k < K; ++k) {
             const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
@@ -561,11 +622,12 @@ struct Searcher {
     }
 };
 
-template <int N, int WPC>
+template <int N, int WPC, bool WIN = false>
 __device__ void search(const PicDev &R, const PicDev &C, const tcme_call &c, tcme_result *res,
-                       tcme_stats *st, Shared<N, WPC> &sh)
+                       tcme_stats *st, anc_t *qw, int *red,
+                       LumaWin lw = LumaWin{nullptr, 0, 0, 0, 0})
 {
-    Searcher<N, WPC> S(R, C, c.px, c.py, sh);
+    Searcher<N, WPC, WIN> S(R, C, c.px, c.py, qw, red, lw);
     // Centre clamp with tc_clip's order: low bound first (motion.c:391-392).
     int bx = c.cx < 0 ? 0 : (c.cx > R.w - N ? R.w - N : c.cx);
     int by = c.cy < 0 ? 0 : (c.cy > R.h - N ? R.h - N : c.cy);
@@ -586,7 +648,7 @@ __device__ void search(const PicDev &R, const PicDev &C, const tcme_call &c, tcm
         qx = bx * 4;
         qy = by * 4;
     }
-    if (threadIdx.x == 0) {
+    if (S.tid == 0) {
         res->mv_x = qx;
         res->mv_y = qy;
         res->sad = best;
@@ -598,6 +660,15 @@ __device__ void search(const PicDev &R, const PicDev &C, const tcme_call &c, tcm
             st->int_oob = S.cnt.oob;
         }
     }
+}
+
+__device__ __forceinline__ void put_status(tcme_result *r, tcme_stats *st, int status)
+{
+    r->mv_x = 0;
+    r->mv_y = 0;
+    r->sad = kInf;
+    r->status = status;
+    if (st) { st->int_evals = st->q6_evals = st->qbil_evals = st->int_oob = 0; }
 }
 
 // Validates one descriptor; returns 0 and the two pictures, or a TCME_E_* code.
@@ -647,16 +718,325 @@ __global__ __launch_bounds__(64 * WPC) void me_search_kernel(const PicDev *__res
     const int status = check_call(table, npics, c, R, C);
     tcme_stats *st = stats ? stats + i : nullptr;
     if (status) {
-        if (threadIdx.x == 0) {
-            res[i].mv_x = 0;
-            res[i].mv_y = 0;
-            res[i].sad = kInf;
-            res[i].status = status;
-            if (st) { st->int_evals = st->q6_evals = st->qbil_evals = st->int_oob = 0; }
-        }
+        if (threadIdx.x == 0) put_status(res + i, st, status);
         return;
     }
-    search<N, WPC>(R, C, c, res + i, st, sh);
+    search<N, WPC>(R, C, c, res + i, st, sh.win, &sh.red[0][0][0]);
+}
+
+// Sum over each group of L consecutive lanes (L = 1, 2, 4, 8 or 16, groups
+// aligned to L); every lane of a group gets its group's total.  DPP
+// butterflies: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
+// row_mirror.  A group's lanes are all active or all inactive.
+template <int L>
+__device__ __forceinline__ int group_sum(int v)
+{
+    if (L >= 2) v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    if (L >= 4) v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    if (L >= 8) v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
+    if (L >= 16) v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
+    return v;
+}
+
+// Full-pel search of 64/L calls of one block size at once: each call is
+// walked by a group of L lanes, lane j of the group holding rows
+// [j*R, j*R + R) of the current block (R = N/L, 16 packed bytes or fewer per
+// row).  The walk is the reference's sequential loop run per lane
+// (reference src/motion.c:384-448), so the walk state and every decision live
+// in vector registers and calls of one wavefront diverge freely; a candidate
+// costs each lane R rows of reference bytes, from the LDS window when the
+// block lies inside it, else from the plane.
+template <int N, int L>
+struct GroupWalk {
+    static constexpr int R = N / L;
+    static constexpr int DW = N / 4;
+    const PicDev &Rf;
+    gptr32 rplane;
+    LumaWin lw;
+    uint32_t o[R * DW];
+    int j;
+
+    __device__ GroupWalk(const PicDev &ref, LumaWin win, const PicDev &cur, int px, int py, int lane)
+        : Rf(ref), rplane(as_global32(ref.y)), lw(win), j(lane % L)
+    {
+        const gptr32 cp = as_global32(cur.y);
+        const size_t base = static_cast<size_t>(py + j * R) * cur.pitch + px;
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr)
+#pragma unroll
+            for (int d = 0; d < DW; ++d)
+                o[rr * DW + d] = ld4g(cp, base + static_cast<size_t>(rr) * cur.pitch + 4 * d);
+    }
+
+    // SAD of the group's block at full-pel (x,y), which must fit in the frame.
+    __device__ __forceinline__ int sad(int x, int y) const
+    {
+        uint32_t acc = 0;
+        if (x >= lw.x0 && y >= lw.y0 && x + N <= lw.x1 && y + N <= lw.y1) {
+            const int a = (y - lw.y0 + j * R) * kWinPitch + (x - lw.x0);
+            const uint32_t sh = static_cast<uint32_t>(a & 3);
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                const uint32_t *q = lw.w + ((a >> 2) + rr * (kWinPitch / 4));
+                uint32_t t[DW + 1];
+#pragma unroll
+                for (int d = 0; d <= DW; ++d) t[d] = q[d];
+#pragma unroll
+                for (int d = 0; d < DW; ++d)
+                    acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(t[d + 1], t[d], sh),
+                                                  o[rr * DW + d], acc);
+            }
+        } else {
+            const size_t b = static_cast<size_t>(y + j * R) * Rf.pitch + x;
+            const uint32_t sh = static_cast<uint32_t>(b & 3);
+#pragma unroll
+            for (int rr = 0; rr < R; ++rr) {
+                const gptr32 q = rplane + ((b >> 2) + static_cast<size_t>(rr) * (Rf.pitch >> 2));
+                uint32_t t[DW + 1];
+#pragma unroll
+                for (int d = 0; d <= DW; ++d) t[d] = q[d];
+#pragma unroll
+                for (int d = 0; d < DW; ++d)
+                    acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(t[d + 1], t[d], sh),
+                                                  o[rr * DW + d], acc);
+            }
+        }
+        return group_sum<L>(static_cast<int>(acc));
+    }
+
+    // One pattern stage, exactly the reference's loop: candidates in table
+    // order around the current best, a strictly better one moves the centre
+    // at once; a pass without a move ends the stage.
+    template <int P>
+    __device__ __forceinline__ void stage(int scale, int passes, int &bx, int &by, int &best,
+                                          int &ie, int &oob) const
+    {
+        for (int it = 0; it < passes; ++it) {
+            bool moved = false;
+#pragma unroll
+            for (int k = 0; k < Pat<P>::K; ++k) {
+                const int x = bx + Pat<P>::x(k) * scale, y = by + Pat<P>::y(k) * scale;
+                if (fits(Rf, N, x, y)) {
+                    ++ie;
+                    const int s = sad(x, y);
+                    if (s < best) { best = s; bx = x; by = y; moved = true; }
+                } else {
+                    ++oob;
+                }
+            }
+            if (!moved) break;
+        }
+    }
+};
+
+// Tile workgroups for the 8x8 and 16x16 calls.  A workgroup takes `chunk`
+// consecutive calls, keeps those of size 8 and 16 in list order, and splits
+// them into runs of consecutive calls that share the reference picture and the
+// 64x64 tile holding their centre.  For each run it stages the reference luma
+// around that tile (kWinMargin samples on each side) in LDS once, cuts the
+// run's calls of each size into slots of 64/L calls, and its wavefronts take
+// slots: the full-pel walks of a slot's calls run side by side (GroupWalk),
+// then the wavefront refines each call in turn at quarter-pel (Searcher::qpel,
+// all 64 lanes on one call).  Candidates outside the window are read from
+// the plane, so any call order gives the same results; lists grouped by
+// (CTU, reference), as an encoder issues them, make one run per chunk.
+constexpr int kL8 = 4;    // lanes per 8x8 call
+constexpr int kL16 = 8;   // lanes per 16x16 call
+
+template <int WAVES>
+struct TileShared {
+    uint32_t luma[kWinWords];
+    anc_t qw[WAVES][18 * 18];        // per-wavefront quarter-pel anchor window (N <= 16)
+    tcme_call call[kTileChunkMax];   // the chunk's 8/16 calls, list order
+    int idx[kTileChunkMax];          // their list positions
+    uint8_t list[2][kTileChunkMax];  // the current run's 16x16 / 8x8 calls (indices into call)
+    int nlist[2];
+    int nlive;
+    int next;                        // slot counter, 64 per slot taken
+};
+
+template <int N, int L, int WAVES>
+__device__ __forceinline__ void run_slot(TileShared<WAVES> &sh, const PicDev *__restrict__ table,
+                                         const PicDev &R, const LumaWin &lw, int first, int cnt,
+                                         int lane, int wave, tcme_result *__restrict__ res,
+                                         tcme_stats *__restrict__ stats)
+{
+    constexpr int S = 64 / L;
+    const int li = N == 16 ? 0 : 1;
+    const int g = lane / L;
+    int bx = 0, by = 0, best = kInf, ie = 0, oob = 0;
+    if (g < cnt) {
+        const tcme_call &c = sh.call[sh.list[li][first + g]];
+        const PicDev C = table[c.cur_pic];
+        GroupWalk<N, L> G(R, lw, C, c.px, c.py, lane);
+        // centre clamp with tc_clip's order, low bound first (motion.c:391-392)
+        bx = c.cx < 0 ? 0 : (c.cx > R.w - N ? R.w - N : c.cx);
+        by = c.cy < 0 ? 0 : (c.cy > R.h - N ? R.h - N : c.cy);
+        if (fits(R, N, bx, by)) { best = G.sad(bx, by); ++ie; }
+        else { ++oob; }
+        if (R.w >= N && R.h >= N) {
+            const int scale = c.sr / 4 > 1 ? c.sr / 4 : 1;
+            G.template stage<PAT_LARGE>(scale, 2, bx, by, best, ie, oob);
+            G.template stage<PAT_SMALL>(1, 3, bx, by, best, ie, oob);
+            G.template stage<PAT_DIAMOND>(1, 4, bx, by, best, ie, oob);
+        } else {
+            oob += 6 + 6 + 4;  // one pass of each stage, every candidate off-frame
+        }
+    }
+    // quarter-pel, one call at a time with the whole wavefront
+    for (int k = 0; k < cnt && k < S; ++k) {
+        const int src = k * L;
+        const int qbx = __builtin_amdgcn_readlane(bx, src);
+        const int qby = __builtin_amdgcn_readlane(by, src);
+        int32_t qbest = __builtin_amdgcn_readlane(best, src);
+        const int k_ie = __builtin_amdgcn_readlane(ie, src);
+        const int k_oob = __builtin_amdgcn_readlane(oob, src);
+        const int ci = sh.list[li][first + k];
+        const tcme_call c = sh.call[ci];
+        const PicDev C = table[c.cur_pic];
+        Searcher<N, 1> Q(R, C, c.px, c.py, &sh.qw[wave][0], nullptr);
+        int qx = qbx * 4, qy = qby * 4;
+        if (R.w >= N && R.h >= N) Q.qpel(qbx, qby, qx, qy, qbest);
+        if (lane == 0) {
+            const int i = sh.idx[ci];
+            res[i].mv_x = qx;
+            res[i].mv_y = qy;
+            res[i].sad = qbest;
+            res[i].status = 0;
+            if (stats) {
+                stats[i].int_evals = k_ie;
+                stats[i].q6_evals = Q.cnt.q6;
+                stats[i].qbil_evals = Q.cnt.qb;
+                stats[i].int_oob = k_oob;
+            }
+        }
+    }
+}
+
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__restrict__ table,
+                                                            int npics,
+                                                            const tcme_call *__restrict__ calls,
+                                                            int n, int chunk,
+                                                            tcme_result *__restrict__ res,
+                                                            tcme_stats *__restrict__ stats)
+{
+    __shared__ TileShared<WAVES> sh;
+    const int b = xcd_slot(blockIdx.x, gridDim.x);
+    const int c0 = b * chunk;
+    if (c0 >= n) return;
+    const int cnt = n - c0 < chunk ? n - c0 : chunk;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    if (wave == 0) {
+        int base = 0;
+        for (int t0 = 0; t0 < cnt; t0 += 64) {
+            const int t = t0 + lane;
+            bool live = false;
+            tcme_call c{};
+            if (t < cnt) {
+                c = calls[c0 + t];
+                if (c.blk == 8 || c.blk == 16) {
+                    PicDev R{}, C{};
+                    const int status = check_call(table, npics, c, R, C);
+                    if (status) put_status(res + c0 + t, stats ? stats + c0 + t : nullptr, status);
+                    else live = true;
+                }
+            }
+            const uint64_t m = __ballot(live);
+            if (live) {
+                const int pos = base + __popcll(m & ((1ull << lane) - 1));
+                sh.call[pos] = c;
+                sh.idx[pos] = c0 + t;
+            }
+            base += __popcll(m);
+        }
+        if (lane == 0) sh.nlive = base;
+    }
+    __syncthreads();
+    const int nlive = sh.nlive;
+    int pos = 0;
+    while (pos < nlive) {
+        const int ref = sh.call[pos].ref_pic;
+        const int tx = sh.call[pos].cx >> 6, ty = sh.call[pos].cy >> 6;
+        // the run ends at the first later call with another reference or tile
+        int end = nlive;
+        for (int p0 = pos + 1; p0 < nlive; p0 += 64) {
+            const int p = p0 + lane;
+            bool brk = false;
+            if (p < nlive) {
+                brk = sh.call[p].ref_pic != ref || (sh.call[p].cx >> 6) != tx ||
+                      (sh.call[p].cy >> 6) != ty;
+            }
+            const uint64_t m = __ballot(brk);
+            if (m) { end = p0 + __builtin_ctzll(m); break; }
+        }
+        const PicDev R = table[ref];
+        LumaWin lw;
+        lw.w = sh.luma;
+        lw.x0 = tx * 64 - kWinMargin < 0 ? 0 : tx * 64 - kWinMargin;
+        lw.y0 = ty * 64 - kWinMargin < 0 ? 0 : ty * 64 - kWinMargin;
+        lw.x1 = tx * 64 + 64 + kWinMargin > R.w ? R.w : tx * 64 + 64 + kWinMargin;
+        lw.y1 = ty * 64 + 64 + kWinMargin > R.h ? R.h : ty * 64 + 64 + kWinMargin;
+        if (lw.x1 > lw.x0 && lw.y1 > lw.y0) {
+            // rows start 8-byte aligned (x0 is a multiple of 8, pitch of 64);
+            // the last dword may run past x1 into the row's padding, the next
+            // row or the plane's tail slack
+            const int dw = (lw.x1 - lw.x0 + 3) >> 2;
+            const int rows = lw.y1 - lw.y0;
+            const gptr32 plane = as_global32(R.y);
+            for (int i = static_cast<int>(threadIdx.x); i < rows * dw; i += 64 * WAVES) {
+                const int r = i / dw, d = i - r * dw;
+                sh.luma[r * (kWinPitch / 4) + d] =
+                    plane[(static_cast<size_t>(lw.y0 + r) * R.pitch + lw.x0) / 4 + d];
+            }
+        } else {
+            lw.x1 = lw.x0;  // nothing staged: every candidate reads the plane
+        }
+        if (wave == 0) {
+            // the run's calls by size, list order
+            int n16 = 0, n8 = 0;
+            for (int p0 = pos; p0 < end; p0 += 64) {
+                const int p = p0 + lane;
+                const bool in = p < end;
+                const bool is16 = in && sh.call[p].blk == 16;
+                const bool is8 = in && !is16;
+                const uint64_t m16 = __ballot(is16), m8 = __ballot(is8);
+                const uint64_t below = (1ull << lane) - 1;
+                if (is16) sh.list[0][n16 + __popcll(m16 & below)] = static_cast<uint8_t>(p);
+                if (is8) sh.list[1][n8 + __popcll(m8 & below)] = static_cast<uint8_t>(p);
+                n16 += __popcll(m16);
+                n8 += __popcll(m8);
+            }
+            if (lane == 0) {
+                sh.nlist[0] = n16;
+                sh.nlist[1] = n8;
+                sh.next = 0;
+            }
+        }
+        __syncthreads();
+        const int n16 = sh.nlist[0], n8 = sh.nlist[1];
+        const int s16 = (n16 + 64 / kL16 - 1) / (64 / kL16);
+        const int s8 = (n8 + 64 / kL8 - 1) / (64 / kL8);
+        for (;;) {
+            // Every lane adds 1, so a fetch advances the counter by 64 and the
+            // slot index is the first lane's old value / 64.  (A fetch under
+            // `if (lane == 0)` followed by a broadcast lets the compiler run
+            // the other lanes through this loop on their own with a stale
+            // index; a uniform add compiles to one LDS atomic, no branch.)
+            const int s = __builtin_amdgcn_readfirstlane(atomicAdd(&sh.next, 1)) >> 6;
+            if (s >= s16 + s8) break;
+            if (s < s16) {
+                const int first = s * (64 / kL16);
+                run_slot<16, kL16>(sh, table, R, lw, first, n16 - first, lane, wave, res, stats);
+            } else {
+                const int first = (s - s16) * (64 / kL8);
+                run_slot<8, kL8>(sh, table, R, lw, first, n8 - first, lane, wave, res, stats);
+            }
+        }
+        __syncthreads();
+        pos = end;
+    }
 }
 
 // Anchor plane of a picture (reference src/motion.c:70-107 for the taps):
@@ -724,6 +1104,20 @@ bool valid_waves(int blk, int waves)
     case 64: return waves == 2 || waves == 4 || waves == 8;
     default: return false;
     }
+}
+
+hipError_t launch_tile(int waves, int chunk, const PicDev *table, int npics, const tcme_call *calls,
+                       int n, tcme_result *res, tcme_stats *stats, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int grid = (n + chunk - 1) / chunk;
+    if (waves == 4)
+        hipLaunchKernelGGL(me_tile_kernel<4>, dim3(grid), dim3(256), 0, s, table, npics, calls, n,
+                           chunk, res, stats);
+    else
+        hipLaunchKernelGGL(me_tile_kernel<8>, dim3(grid), dim3(512), 0, s, table, npics, calls, n,
+                           chunk, res, stats);
+    return hipGetLastError();
 }
 
 hipError_t launch_search_n(int blk, int waves, const PicDev *table, int npics,

@@ -3,7 +3,9 @@
 v_ashr_pk_u8_i32 writes only the low half of its destination on the device,
 while the compiler ORs further bytes into the upper half as if it were zero;
 the library is built with that instruction disabled and must not contain it.
-LDS accesses stay naturally aligned (anchors are read as whole 8-byte words)."""
+LDS accesses stay naturally aligned (anchors are read as whole 8-byte words),
+and no access goes through flat (generic-address) instructions: every plane
+read is a global load and every window read an LDS one."""
 import os
 import re
 import subprocess
@@ -40,8 +42,16 @@ def test_no_packed_shift_clip():
     assert "v_ashr_pk_" not in text
 
 
+def test_no_flat_memory_instructions():
+    text = assembly()
+    flat = re.findall(r"\n\s+(flat_\w+)", text)
+    assert not flat, sorted(set(flat))
+
+
 def test_every_kernel_present():
     text = assembly()
     for n, wpc in [(8, 1), (16, 1), (32, 1), (32, 2), (32, 4), (64, 2), (64, 4), (64, 8)]:
         assert re.search(rf"me_search_kernelILi{n}ELi{wpc}E", text), (n, wpc)
+    for w in (4, 8):
+        assert re.search(rf"me_tile_kernelILi{w}E", text), w
     assert "anchor_kernel" in text

@@ -125,6 +125,12 @@ def test_gpu_rejects_bad_tiling(engine):
         engine.set_tiling(1, 1, 3, 4)
     with pytest.raises(me.TcmeError):
         engine.set_tiling(1, 1, 2, 1)
+    with pytest.raises(me.TcmeError):
+        engine.set_tiling(1, 1, 2, 4, 3, 85)
+    with pytest.raises(me.TcmeError):
+        engine.set_tiling(1, 1, 2, 4, 8, 0)
+    with pytest.raises(me.TcmeError):
+        engine.set_tiling(1, 1, 2, 4, 4, 129)
 
 
 def test_gpu_rejects_bad_descriptors(engine):
@@ -188,12 +194,16 @@ def test_dropin_symbol_matches_oracle(orc):
         assert (mx, my, sad) == orc.motion_est(new, cur, 0, 0, 40, 40, 16, 24)
 
 
-@pytest.mark.parametrize("tiling", [(1, 1, 1, 2), (1, 1, 2, 4), (1, 1, 4, 8)],
-                         ids=lambda t: "w%d_%d_%d_%d" % t)
+TILINGS = [(1, 1, 1, 2, 0, 85), (1, 1, 2, 4, 4, 85), (1, 1, 4, 8, 8, 85), (1, 1, 1, 4, 8, 1),
+           (1, 1, 1, 4, 8, 37), (1, 1, 1, 4, 4, 128)]
+
+
+@pytest.mark.parametrize("tiling", TILINGS, ids=lambda t: "w%d_%d_%d_%d_t%d_c%d" % t)
 def test_gpu_tilings_agree_with_oracle(engine, orc, tiling):
-    """Every launch shape (wavefronts per 32x32 and 64x64 call) gives the
-    oracle's answers and candidate counts, for CTU-major and reference-major
-    call orders."""
+    """Every launch shape (wavefronts per 32x32 and 64x64 call; tile
+    workgroups of 0, 4 or 8 wavefronts and 1..128 calls for the 8x8/16x16
+    calls) gives the oracle's answers and candidate counts, for CTU-major,
+    reference-major and shuffled call orders."""
     from tcodec_amd import synth, workload
     log(f"tiling {tiling}")
     fr = synth.scene_video(256, 200, 5, seed=9, rects=6, pan=(0.75, 0.25))
@@ -204,11 +214,14 @@ def test_gpu_tilings_agree_with_oracle(engine, orc, tiling):
     old = engine.tiling()
     engine.set_tiling(*tiling)
     try:
-        for order in ("ctu", "ref"):
+        for order in ("ctu", "ref", "shuffled"):
             if order == "ctu":
                 calls = workload.frame_calls_by_ctu(256, 200, rps, cp, workload.SR_B)
-            else:
+            elif order == "ref":
                 calls = workload.frame_calls(workload.qt_nodes(256, 200), rps, cp, workload.SR_P)
+            else:
+                calls = workload.frame_calls_by_ctu(256, 200, rps, cp, workload.SR_B)
+                calls = calls[np.random.default_rng(3).permutation(len(calls))]
             res, st = engine.search(calls, stats=True)
             for k, rp in enumerate(rps):
                 m = calls["ref_pic"] == rp
@@ -217,7 +230,7 @@ def test_gpu_tilings_agree_with_oracle(engine, orc, tiling):
                 assert np.array_equal(st["int_evals"][m], wst[:, 0])
                 assert np.array_equal(st["q6_evals"][m], wst[:, 2])
     finally:
-        engine.set_tiling(old.waves8, old.waves16, old.waves32, old.waves64)
+        engine.set_tiling(**old.as_dict())
         for p in rps + [cp]:
             engine.free(p)
 
@@ -238,12 +251,13 @@ def test_gpu_autotune_keeps_results(engine, orc):
         best, ms = engine.autotune(d_calls.data_ptr(), len(calls), d_res.data_ptr(), reps=1,
                                    stream=torch.cuda.current_stream().cuda_stream)
         assert ms > 0 and best.waves32 in (1, 2, 4) and best.waves64 in (2, 4, 8)
+        assert best.tile_waves in (0, 4, 8)
         assert engine.tiling().as_dict() == best.as_dict()
         torch.cuda.synchronize()
         got = d_res.cpu().numpy().view(me.RESULT_DTYPE).ravel()
         compare(got, oracle_batch(orc, ref, cur, calls), calls, "autotune")
         compare(engine.search(calls), oracle_batch(orc, ref, cur, calls), calls, "after tune")
     finally:
-        engine.set_tiling(old.waves8, old.waves16, old.waves32, old.waves64)
+        engine.set_tiling(**old.as_dict())
         engine.free(rp)
         engine.free(cp)
