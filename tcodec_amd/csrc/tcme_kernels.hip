@@ -581,6 +581,24 @@ struct Searcher {
         }
     }
 
+    // Quarter-pel refinement from the planes, every candidate priced with its
+    // own class (six-tap or bilinear) straight from the anchor and luma planes.
+    __device__ __forceinline__ void qpel_direct(int bx, int by, int &qx, int &qy, int32_t &best)
+    {
+        constexpr int K = Pat<PAT_QPEL>::K;
+        qx = bx * 4;
+        qy = by * 4;
+#pragma unroll 1
+        for (int k = 0; k < K; ++k) {
+            const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
+            const int cls = qclass(R, N, tx, ty);
+            if (!cls) continue;
+            if (cls == 1) cnt.q6++; else cnt.qb++;
+            const int32_t sv = qsad_direct(tx, ty, cls);
+            if (sv < best) { best = sv; qx = tx; qy = ty; }
+        }
+    }
+
     // Quarter-pel refinement, one greedy pass in the reference's order
     // (reference src/motion.c:450-481).  (bx,by) is the integer best.  When
     // every sample position the pass can reach (fx in {bx-1, bx}, fy in
@@ -594,15 +612,7 @@ struct Searcher {
         qy = by * 4;
         const bool fast = bx - 1 >= 2 && by - 1 >= 2 && bx + N + 2 < R.w && by + N + 2 < R.h;
         if (!fast) {
-#pragma unroll 1
-            for (int k = 0; k < K; ++k) {
-                const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
-                const int cls = qclass(R, N, tx, ty);
-                if (!cls) continue;
-                if (cls == 1) cnt.q6++; else cnt.qb++;
-                const int32_t sv = qsad_direct(tx, ty, cls);
-                if (sv < best) { best = sv; qx = tx; qy = ty; }
-            }
+            qpel_direct(bx, by, qx, qy, best);
             return;
         }
         const int x0 = bx - 1, y0 = by - 1;  // window origin (sample coordinates)
@@ -738,14 +748,40 @@ __device__ __forceinline__ int group_sum(int v)
     return v;
 }
 
-// Full-pel search of 64/L calls of one block size at once: each call is
-// walked by a group of L lanes, lane j of the group holding rows
-// [j*R, j*R + R) of the current block (R = N/L, 16 packed bytes or fewer per
-// row).  The walk is the reference's sequential loop run per lane
-// (reference src/motion.c:384-448), so the walk state and every decision live
-// in vector registers and calls of one wavefront diverge freely; a candidate
-// costs each lane R rows of reference bytes, from the LDS window when the
-// block lies inside it, else from the plane.
+// Half-sample lattice of the reference around a 64x64 tile, staged in LDS
+// from the anchor plane: anchors [x0,x1) x [y0,y1) (at most kLatSide per
+// side), lattice row 2r holding the low anchor words of anchor row r
+// (sample | hh << 16) and row 2r+1 the high words (vv | cc << 16), one dword
+// per anchor, kLatPitch dwords per row.  Lattice entry (Y, X), X counting
+// 16-bit halves, is then the sample, hh, vv or cc at half-sample position
+// (X/2, Y/2) for (Y, X) even/even, even/odd, odd/even, odd/odd.
+constexpr int kLatMargin = 8;
+constexpr int kLatSide = 64 + 2 * kLatMargin;   // 80 anchors
+constexpr int kLatPitch = kLatSide + 1;         // 81 dwords per lattice row (odd: banks)
+constexpr int kLatWords = 2 * kLatSide * kLatPitch;
+
+struct LatWin {
+    const uint32_t *w;  // LDS
+    int x0, y0, x1, y1;
+};
+
+// The signed 16-bit field of v starting at bit `bit` (0 or 16).  The
+// builtin's result type is unsigned: it is converted before any arithmetic,
+// or sums of negative fields would wrap and shift logically.
+__device__ __forceinline__ int half16(uint32_t v, int bit)
+{
+    return static_cast<int>(__builtin_amdgcn_sbfe(static_cast<int>(v), bit, 16));
+}
+
+// Full-pel search and quarter-pel refinement of 64/L calls of one block size
+// at once: each call is walked by a group of L lanes, lane j of the group
+// holding rows [j*R, j*R + R) of the current block (R = N/L).  The walks are
+// the reference's sequential loops run per lane (reference
+// src/motion.c:384-481), so the walk state and every decision live in vector
+// registers and the calls of one wavefront diverge freely.  A full-pel
+// candidate costs each lane R rows of reference bytes, from the LDS luma
+// window when the block lies inside it, else from the plane; a quarter-pel
+// candidate costs R rows of lattice words from LDS.
 template <int N, int L>
 struct GroupWalk {
     static constexpr int R = N / L;
@@ -827,46 +863,131 @@ struct GroupWalk {
             if (!moved) break;
         }
     }
+
+    // Whether the quarter-pel pass around the full-pel best (bx,by) can run
+    // from the lattice: every candidate six-tap (the reference's class test,
+    // src/motion.c:455-464) and every anchor it reads, [bx-1, bx+N] x
+    // [by-1, by+N], inside the lattice window.
+    __device__ __forceinline__ bool lattice_ok(const LatWin &lt, int bx, int by) const
+    {
+        return bx - 1 >= 2 && by - 1 >= 2 && bx + N + 2 < Rf.w && by + N + 2 < Rf.h &&
+               bx - 1 >= lt.x0 && by - 1 >= lt.y0 && bx + N < lt.x1 && by + N < lt.y1;
+    }
+
+    // SAD of the quarter-pel candidate (tx,ty) from the lattice.  Sample
+    // (c,r) of the block sits at quarter-pel (tx + 4c, ty + 4r); with
+    // X0 = q >> 1 and X1 = (q + 1) >> 1 per axis its value is
+    //   clip((avg(M[Y0][X0], M[Y0][X1]) + avg(M[Y1][X0], M[Y1][X1]) + 1) >> 1),
+    // avg(a,b) = (a + b + 1) >> 1, which is the six-tap quarter sample of
+    // the reference (src/motion.c:193-260) for every phase: X0 == X1 (or
+    // Y0 == Y1) on the full- and half-sample phases.
+    __device__ __forceinline__ int qsad(const LatWin &lt, int tx, int ty) const
+    {
+        const int cx = (tx >> 1) - 2 * lt.x0;           // lattice column of X0 at c = 0
+        const int s16 = (((tx + 1) >> 1) - (tx >> 1)) * 16;  // X1 - X0, in bits
+        const uint32_t hs = static_cast<uint32_t>(cx & 1) * 16;
+        const int y0 = (ty >> 1) - 2 * lt.y0 + 2 * j * R;
+        const int y1 = ((ty + 1) >> 1) - 2 * lt.y0 + 2 * j * R;
+        uint32_t acc = 0;
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            const uint32_t *p0 = lt.w + (y0 + 2 * rr) * kLatPitch + (cx >> 1);
+            const uint32_t *p1 = lt.w + (y1 + 2 * rr) * kLatPitch + (cx >> 1);
+#pragma unroll
+            for (int d = 0; d < DW; ++d) {
+                uint32_t a[5], b[5];
+#pragma unroll
+                for (int t = 0; t < 5; ++t) {
+                    a[t] = p0[4 * d + t];
+                    b[t] = p1[4 * d + t];
+                }
+                uint32_t pk = 0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    // the two halves from X0 on: M[.][X0] low, M[.][X0 + 1] high
+                    const uint32_t da = __builtin_amdgcn_alignbit(a[t + 1], a[t], hs);
+                    const uint32_t db = __builtin_amdgcn_alignbit(b[t + 1], b[t], hs);
+                    const int top = (half16(da, 0) + half16(da, s16) + 1) >> 1;
+                    const int bot = (half16(db, 0) + half16(db, s16) + 1) >> 1;
+                    int v = (top + bot + 1) >> 1;
+                    v = v < 0 ? 0 : (v > 255 ? 255 : v);
+                    pk |= static_cast<uint32_t>(v) << (8 * t);
+                }
+                acc = __builtin_amdgcn_sad_u8(pk, o[rr * DW + d], acc);
+            }
+        }
+        return group_sum<L>(static_cast<int>(acc));
+    }
+
+    // The quarter-pel pass from the lattice (lattice_ok must hold): every
+    // candidate is six-tap and in frame.
+    __device__ __forceinline__ void qpel(const LatWin &lt, int bx, int by, int &qx, int &qy,
+                                         int &best, int &q6) const
+    {
+        qx = bx * 4;
+        qy = by * 4;
+#pragma unroll
+        for (int k = 0; k < Pat<PAT_QPEL>::K; ++k) {
+            const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
+            ++q6;
+            const int s = qsad(lt, tx, ty);
+            if (s < best) { best = s; qx = tx; qy = ty; }
+        }
+    }
 };
 
 // Tile workgroups for the 8x8 and 16x16 calls.  A workgroup takes `chunk`
 // consecutive calls, keeps those of size 8 and 16 in list order, and splits
 // them into runs of consecutive calls that share the reference picture and the
-// 64x64 tile holding their centre.  For each run it stages the reference luma
-// around that tile (kWinMargin samples on each side) in LDS once, cuts the
-// run's calls of each size into slots of 64/L calls, and its wavefronts take
-// slots: the full-pel walks of a slot's calls run side by side (GroupWalk),
-// then the wavefront refines each call in turn at quarter-pel (Searcher::qpel,
-// all 64 lanes on one call).  Candidates outside the window are read from
-// the plane, so any call order gives the same results; lists grouped by
-// (CTU, reference), as an encoder issues them, make one run per chunk.
+// 64x64 tile holding their centre.  For each run it stages in LDS, once, the
+// reference luma around that tile (kWinMargin samples on each side) and the
+// half-sample lattice around it (kLatMargin), cuts the run's calls of each
+// size into slots of 64/L calls, and its wavefronts take slots: a slot's calls
+// are searched side by side (GroupWalk).  Candidates outside the luma window
+// are read from the plane.  Calls whose quarter-pel pass leaves the lattice
+// (or needs bilinear edge candidates) are deferred: once every slot is done,
+// the lattice's LDS holds one anchor window per wavefront and the wavefronts
+// refine the deferred calls one at a time (Searcher::qpel).  Any call order
+// gives the same results; lists grouped by (CTU, reference), as an encoder
+// issues them, make one run per chunk.
 constexpr int kL8 = 4;    // lanes per 8x8 call
 constexpr int kL16 = 8;   // lanes per 16x16 call
 
 template <int WAVES>
 struct TileShared {
     uint32_t luma[kWinWords];
-    anc_t qw[WAVES][18 * 18];        // per-wavefront quarter-pel anchor window (N <= 16)
+    union {
+        uint32_t lat[kLatWords];     // the run's lattice, while slots run
+        anc_t qw[WAVES][18 * 18];    // then per-wavefront anchor windows for deferred calls
+    };
     tcme_call call[kTileChunkMax];   // the chunk's 8/16 calls, list order
     int idx[kTileChunkMax];          // their list positions
     uint8_t list[2][kTileChunkMax];  // the current run's 16x16 / 8x8 calls (indices into call)
+    // calls whose quarter-pel pass leaves the lattice: call index and
+    // full-pel state (x, y, SAD, counts)
+    int def_ci[kTileChunkMax], def_x[kTileChunkMax], def_y[kTileChunkMax];
+    int def_sad[kTileChunkMax], def_ie[kTileChunkMax], def_oob[kTileChunkMax];
     int nlist[2];
+    int ndef;
     int nlive;
     int next;                        // slot counter, 64 per slot taken
 };
 
 template <int N, int L, int WAVES>
 __device__ __forceinline__ void run_slot(TileShared<WAVES> &sh, const PicDev *__restrict__ table,
-                                         const PicDev &R, const LumaWin &lw, int first, int cnt,
-                                         int lane, int wave, tcme_result *__restrict__ res,
+                                         const PicDev &R, const LumaWin &lw, const LatWin &lt,
+                                         int first, int cnt, int lane,
+                                         tcme_result *__restrict__ res,
                                          tcme_stats *__restrict__ stats)
 {
     constexpr int S = 64 / L;
     const int li = N == 16 ? 0 : 1;
     const int g = lane / L;
+    bool slow = false;  // this lane's call needs the quarter-pel pass from the planes
     int bx = 0, by = 0, best = kInf, ie = 0, oob = 0;
     if (g < cnt) {
-        const tcme_call &c = sh.call[sh.list[li][first + g]];
+        const int ci = sh.list[li][first + g];
+        const tcme_call &c = sh.call[ci];
         const PicDev C = table[c.cur_pic];
         GroupWalk<N, L> G(R, lw, C, c.px, c.py, lane);
         // centre clamp with tc_clip's order, low bound first (motion.c:391-392)
@@ -874,43 +995,42 @@ __device__ __forceinline__ void run_slot(TileShared<WAVES> &sh, const PicDev *__
         by = c.cy < 0 ? 0 : (c.cy > R.h - N ? R.h - N : c.cy);
         if (fits(R, N, bx, by)) { best = G.sad(bx, by); ++ie; }
         else { ++oob; }
+        int qx = bx * 4, qy = by * 4, q6 = 0;
         if (R.w >= N && R.h >= N) {
             const int scale = c.sr / 4 > 1 ? c.sr / 4 : 1;
             G.template stage<PAT_LARGE>(scale, 2, bx, by, best, ie, oob);
             G.template stage<PAT_SMALL>(1, 3, bx, by, best, ie, oob);
             G.template stage<PAT_DIAMOND>(1, 4, bx, by, best, ie, oob);
+            if (G.lattice_ok(lt, bx, by)) G.qpel(lt, bx, by, qx, qy, best, q6);
+            else slow = true;
         } else {
             oob += 6 + 6 + 4;  // one pass of each stage, every candidate off-frame
         }
-    }
-    // quarter-pel, one call at a time with the whole wavefront
-    for (int k = 0; k < cnt && k < S; ++k) {
-        const int src = k * L;
-        const int qbx = __builtin_amdgcn_readlane(bx, src);
-        const int qby = __builtin_amdgcn_readlane(by, src);
-        int32_t qbest = __builtin_amdgcn_readlane(best, src);
-        const int k_ie = __builtin_amdgcn_readlane(ie, src);
-        const int k_oob = __builtin_amdgcn_readlane(oob, src);
-        const int ci = sh.list[li][first + k];
-        const tcme_call c = sh.call[ci];
-        const PicDev C = table[c.cur_pic];
-        Searcher<N, 1> Q(R, C, c.px, c.py, &sh.qw[wave][0], nullptr);
-        int qx = qbx * 4, qy = qby * 4;
-        if (R.w >= N && R.h >= N) Q.qpel(qbx, qby, qx, qy, qbest);
-        if (lane == 0) {
+        if (!slow && G.j == 0) {
             const int i = sh.idx[ci];
             res[i].mv_x = qx;
             res[i].mv_y = qy;
-            res[i].sad = qbest;
+            res[i].sad = best;
             res[i].status = 0;
             if (stats) {
-                stats[i].int_evals = k_ie;
-                stats[i].q6_evals = Q.cnt.q6;
-                stats[i].qbil_evals = Q.cnt.qb;
-                stats[i].int_oob = k_oob;
+                stats[i].int_evals = ie;
+                stats[i].q6_evals = q6;
+                stats[i].qbil_evals = 0;
+                stats[i].int_oob = oob;
             }
         }
     }
+    if (slow && lane % L == 0) {
+        // a per-lane slot (no broadcast of the result): safe under divergence
+        const int d = atomicAdd(&sh.ndef, 1);
+        sh.def_ci[d] = sh.list[li][first + g];
+        sh.def_x[d] = bx;
+        sh.def_y[d] = by;
+        sh.def_sad[d] = best;
+        sh.def_ie[d] = ie;
+        sh.def_oob[d] = oob;
+    }
+    (void)S;
 }
 
 template <int WAVES>
@@ -993,6 +1113,24 @@ __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__res
         } else {
             lw.x1 = lw.x0;  // nothing staged: every candidate reads the plane
         }
+        LatWin lt;
+        lt.w = sh.lat;
+        lt.x0 = tx * 64 - kLatMargin < 0 ? 0 : tx * 64 - kLatMargin;
+        lt.y0 = ty * 64 - kLatMargin < 0 ? 0 : ty * 64 - kLatMargin;
+        lt.x1 = tx * 64 + 64 + kLatMargin > R.w ? R.w : tx * 64 + 64 + kLatMargin;
+        lt.y1 = ty * 64 + 64 + kLatMargin > R.h ? R.h : ty * 64 + 64 + kLatMargin;
+        if (lt.x1 > lt.x0 && lt.y1 > lt.y0) {
+            const int cols = lt.x1 - lt.x0, rows = lt.y1 - lt.y0;
+            const gptr64 an = as_global64(R.an);
+            for (int i = static_cast<int>(threadIdx.x); i < rows * cols; i += 64 * WAVES) {
+                const int r = i / cols, c = i - r * cols;
+                const uint64_t a = an[static_cast<size_t>(lt.y0 + r) * R.apitch + lt.x0 + c];
+                sh.lat[(2 * r) * kLatPitch + c] = static_cast<uint32_t>(a);
+                sh.lat[(2 * r + 1) * kLatPitch + c] = static_cast<uint32_t>(a >> 32);
+            }
+        } else {
+            lt.x1 = lt.x0;  // empty: every call takes the plane path
+        }
         if (wave == 0) {
             // the run's calls by size, list order
             int n16 = 0, n8 = 0;
@@ -1012,6 +1150,7 @@ __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__res
                 sh.nlist[0] = n16;
                 sh.nlist[1] = n8;
                 sh.next = 0;
+                sh.ndef = 0;
             }
         }
         __syncthreads();
@@ -1028,13 +1167,51 @@ __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__res
             if (s >= s16 + s8) break;
             if (s < s16) {
                 const int first = s * (64 / kL16);
-                run_slot<16, kL16>(sh, table, R, lw, first, n16 - first, lane, wave, res, stats);
+                run_slot<16, kL16>(sh, table, R, lw, lt, first, n16 - first, lane, res, stats);
             } else {
                 const int first = (s - s16) * (64 / kL8);
-                run_slot<8, kL8>(sh, table, R, lw, first, n8 - first, lane, wave, res, stats);
+                run_slot<8, kL8>(sh, table, R, lw, lt, first, n8 - first, lane, res, stats);
             }
         }
         __syncthreads();
+        // deferred quarter-pel passes, the lattice LDS now reused as windows
+        const int ndef = sh.ndef;
+        if (ndef > 0) {
+            if (threadIdx.x == 0) sh.next = 0;
+            __syncthreads();
+            for (;;) {
+                const int d = __builtin_amdgcn_readfirstlane(atomicAdd(&sh.next, 1)) >> 6;
+                if (d >= ndef) break;
+                const tcme_call c = sh.call[sh.def_ci[d]];
+                const PicDev C = table[c.cur_pic];
+                const int i = sh.idx[sh.def_ci[d]];
+                int32_t best = sh.def_sad[d];
+                int qx, qy;
+                Counts cnt;
+                if (c.blk == 16) {
+                    Searcher<16, 1> Q(R, C, c.px, c.py, &sh.qw[wave][0], nullptr);
+                    Q.qpel(sh.def_x[d], sh.def_y[d], qx, qy, best);
+                    cnt = Q.cnt;
+                } else {
+                    Searcher<8, 1> Q(R, C, c.px, c.py, &sh.qw[wave][0], nullptr);
+                    Q.qpel(sh.def_x[d], sh.def_y[d], qx, qy, best);
+                    cnt = Q.cnt;
+                }
+                if (lane == 0) {
+                    res[i].mv_x = qx;
+                    res[i].mv_y = qy;
+                    res[i].sad = best;
+                    res[i].status = 0;
+                    if (stats) {
+                        stats[i].int_evals = sh.def_ie[d];
+                        stats[i].q6_evals = cnt.q6;
+                        stats[i].qbil_evals = cnt.qb;
+                        stats[i].int_oob = sh.def_oob[d];
+                    }
+                }
+            }
+            __syncthreads();
+        }
         pos = end;
     }
 }
