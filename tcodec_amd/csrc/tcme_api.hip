@@ -151,10 +151,9 @@ int pic_free(tcme_ctx *c, int id)
     return TCME_OK;
 }
 
-// Launches the search of n calls in launch shape t on stream s: the kernels
-// (per-size for 64x64 and 32x32; the tile kernel or per-size ones for 16x16
-// and 8x8) run concurrently on the context's streams, forked from and joined
-// back into s with events.
+// Launches the search of n calls in launch shape t on stream s: the tile
+// kernel alone, or the four per-size kernels concurrently on the context's
+// streams, forked from and joined back into s with events.
 int dispatch(tcme_ctx *c, const tcme_call *calls, int n, const tcme::Tiling &t,
              tcme_result *res, tcme_stats *stats, hipStream_t s)
 {
@@ -162,16 +161,17 @@ int dispatch(tcme_ctx *c, const tcme_call *calls, int n, const tcme::Tiling &t,
     std::lock_guard<std::mutex> g(c->launch_mu);
     static const int sizes[4] = {64, 32, 16, 8};
     static const int slot[4] = {3, 2, 1, 0};
-    const int nk = t.tile_waves ? 3 : 4;
+    if (t.tile_waves) {
+        // one launch: the tile kernel takes every call
+        HIP_TRY(tcme::launch_tile(t.tile_waves, t.tile_chunk, c->d_table, c->max_pics, calls, n,
+                                  res, stats, s));
+        return TCME_OK;
+    }
     HIP_TRY(hipEventRecord(c->fork, s));
-    for (int k = 0; k < nk; ++k) {
+    for (int k = 0; k < 4; ++k) {
         HIP_TRY(hipStreamWaitEvent(c->sub[k], c->fork, 0));
-        if (t.tile_waves && k == 2)
-            HIP_TRY(tcme::launch_tile(t.tile_waves, t.tile_chunk, c->d_table, c->max_pics, calls,
+        HIP_TRY(tcme::launch_search_n(sizes[k], t.waves[slot[k]], c->d_table, c->max_pics, calls,
                                       n, res, stats, c->sub[k]));
-        else
-            HIP_TRY(tcme::launch_search_n(sizes[k], t.waves[slot[k]], c->d_table, c->max_pics,
-                                          calls, n, res, stats, c->sub[k]));
         HIP_TRY(hipEventRecord(c->join[k], c->sub[k]));
         HIP_TRY(hipStreamWaitEvent(s, c->join[k], 0));
     }

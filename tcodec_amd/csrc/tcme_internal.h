@@ -23,7 +23,7 @@ struct PicDev {
 
 constexpr int kTailSlack = 256;
 constexpr int kMaxBlk = 64;
-constexpr int kTileChunkMax = 128;
+constexpr int kTileChunkMax = 96;
 
 // Launch shape of the search (chosen by tcme_autotune or set by the caller):
 // wavefronts per call for each block size (8x8 and 16x16: 1; 32x32: 1, 2

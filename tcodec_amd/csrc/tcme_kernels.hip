@@ -553,6 +553,7 @@ struct Searcher {
                                 (static_cast<uint32_t>(qpix<DX, DY>(e + 2, Gm::WIN)) << 16) |
                                 (static_cast<uint32_t>(qpix<DX, DY>(e + 3, Gm::WIN)) << 24);
             acc = __builtin_amdgcn_sad_u8(pk, o[u], acc);
+            if (Gm::U >= 8) __builtin_amdgcn_sched_barrier(0);
         }
         int32_t v[1] = {wave_sum(static_cast<int>(acc))};
         block_sum<1>(v);
@@ -734,10 +735,11 @@ __global__ __launch_bounds__(64 * WPC) void me_search_kernel(const PicDev *__res
     search<N, WPC>(R, C, c, res + i, st, sh.win, &sh.red[0][0][0]);
 }
 
-// Sum over each group of L consecutive lanes (L = 1, 2, 4, 8 or 16, groups
+// Sum over each group of L consecutive lanes (L = 1, 2, 4, ..., 64, groups
 // aligned to L); every lane of a group gets its group's total.  DPP
 // butterflies: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
-// row_mirror.  A group's lanes are all active or all inactive.
+// row_mirror, then lane permutes across rows.  A group's lanes are all active
+// or all inactive.
 template <int L>
 __device__ __forceinline__ int group_sum(int v)
 {
@@ -745,6 +747,10 @@ __device__ __forceinline__ int group_sum(int v)
     if (L >= 4) v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
     if (L >= 8) v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
     if (L >= 16) v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
+    // across rows: ds_bpermute from lane ^ 16, then lane ^ 32 (no LDS memory)
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    if (L >= 32) v += __builtin_amdgcn_ds_bpermute((lane ^ 16) << 2, v);
+    if (L >= 64) v += __builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, v);
     return v;
 }
 
@@ -914,6 +920,10 @@ struct GroupWalk {
                     pk |= static_cast<uint32_t>(v) << (8 * t);
                 }
                 acc = __builtin_amdgcn_sad_u8(pk, o[rr * DW + d], acc);
+                // keep each group of four samples' loads with their use
+                // (bounds the registers in flight; other wavefronts hide
+                // the LDS latency)
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         return group_sum<L>(static_cast<int>(acc));
@@ -950,40 +960,46 @@ struct GroupWalk {
 // refine the deferred calls one at a time (Searcher::qpel).  Any call order
 // gives the same results; lists grouped by (CTU, reference), as an encoder
 // issues them, make one run per chunk.
-constexpr int kL8 = 4;    // lanes per 8x8 call
-constexpr int kL16 = 8;   // lanes per 16x16 call
+// Lanes per call by block size: 64x64 one call per wavefront, 32x32 four,
+// 16x16 eight, 8x8 sixteen (each lane then holds 64, 64, 32 or 16 samples of
+// its block).
+template <int N> struct Lanes { static constexpr int L = N == 64 ? 64 : N == 32 ? 16 : N == 16 ? 8 : 4; };
+__host__ __device__ constexpr int size_class(int n) { return n == 64 ? 0 : n == 32 ? 1 : n == 16 ? 2 : 3; }
 
 template <int WAVES>
 struct TileShared {
     uint32_t luma[kWinWords];
     union {
         uint32_t lat[kLatWords];     // the run's lattice, while slots run
-        anc_t qw[WAVES][18 * 18];    // then per-wavefront anchor windows for deferred calls
+        anc_t qw[kLatWords / 2];     // then anchor windows for deferred calls
     };
-    tcme_call call[kTileChunkMax];   // the chunk's 8/16 calls, list order
+    tcme_call call[kTileChunkMax];   // the chunk's calls, list order
     int idx[kTileChunkMax];          // their list positions
-    uint8_t list[2][kTileChunkMax];  // the current run's 16x16 / 8x8 calls (indices into call)
-    // calls whose quarter-pel pass leaves the lattice: call index and
-    // full-pel state (x, y, SAD, counts)
-    int def_ci[kTileChunkMax], def_x[kTileChunkMax], def_y[kTileChunkMax];
-    int def_sad[kTileChunkMax], def_ie[kTileChunkMax], def_oob[kTileChunkMax];
-    int nlist[2];
-    int ndef;
+    uint8_t list[4][kTileChunkMax];  // the current run's calls by size (64, 32, 16, 8)
+    // calls whose quarter-pel pass leaves the lattice, by size (64, 32, 8/16):
+    // call index and full-pel state
+    uint8_t def_ci[3][kTileChunkMax];
+    int16_t def_x[3][kTileChunkMax], def_y[3][kTileChunkMax];
+    int def_sad[3][kTileChunkMax];
+    uint8_t def_ie[3][kTileChunkMax], def_oob[3][kTileChunkMax];
+    int nlist[4];
+    int ndef[3];
     int nlive;
-    int next;                        // slot counter, 64 per slot taken
+    int next;                        // work counter, 64 per item taken
 };
 
-template <int N, int L, int WAVES>
+template <int N, int WAVES>
 __device__ __forceinline__ void run_slot(TileShared<WAVES> &sh, const PicDev *__restrict__ table,
                                          const PicDev &R, const LumaWin &lw, const LatWin &lt,
                                          int first, int cnt, int lane,
                                          tcme_result *__restrict__ res,
                                          tcme_stats *__restrict__ stats)
 {
-    constexpr int S = 64 / L;
-    const int li = N == 16 ? 0 : 1;
+    constexpr int L = Lanes<N>::L;
+    constexpr int li = size_class(N);
+    constexpr int di = N == 64 ? 0 : N == 32 ? 1 : 2;
     const int g = lane / L;
-    bool slow = false;  // this lane's call needs the quarter-pel pass from the planes
+    bool slow = false;  // the quarter-pel pass leaves the lattice: deferred
     int bx = 0, by = 0, best = kInf, ie = 0, oob = 0;
     if (g < cnt) {
         const int ci = sh.list[li][first + g];
@@ -1022,15 +1038,54 @@ __device__ __forceinline__ void run_slot(TileShared<WAVES> &sh, const PicDev *__
     }
     if (slow && lane % L == 0) {
         // a per-lane slot (no broadcast of the result): safe under divergence
-        const int d = atomicAdd(&sh.ndef, 1);
-        sh.def_ci[d] = sh.list[li][first + g];
-        sh.def_x[d] = bx;
-        sh.def_y[d] = by;
-        sh.def_sad[d] = best;
-        sh.def_ie[d] = ie;
-        sh.def_oob[d] = oob;
+        const int d = atomicAdd(&sh.ndef[di], 1);
+        sh.def_ci[di][d] = static_cast<uint8_t>(sh.list[li][first + g]);
+        sh.def_x[di][d] = static_cast<int16_t>(bx);
+        sh.def_y[di][d] = static_cast<int16_t>(by);
+        sh.def_sad[di][d] = best;
+        sh.def_ie[di][d] = static_cast<uint8_t>(ie);
+        sh.def_oob[di][d] = static_cast<uint8_t>(oob);
     }
-    (void)S;
+}
+
+// Quarter-pel pass of deferred call d (size class di) by one wavefront, with
+// its (N+2)^2 anchor window at qw.
+template <int N, int WAVES>
+__device__ __forceinline__ void run_deferred(TileShared<WAVES> &sh, const PicDev *__restrict__ table,
+                                             const PicDev &R, int di, int d, anc_t *qw, int lane,
+                                             tcme_result *__restrict__ res,
+                                             tcme_stats *__restrict__ stats)
+{
+    const int ci = sh.def_ci[di][d];
+    const tcme_call c = sh.call[ci];
+    const PicDev C = table[c.cur_pic];
+    Searcher<N, 1> Q(R, C, c.px, c.py, qw, nullptr);
+    int32_t best = sh.def_sad[di][d];
+    int qx, qy;
+    Q.qpel(sh.def_x[di][d], sh.def_y[di][d], qx, qy, best);
+    if (lane == 0) {
+        const int i = sh.idx[ci];
+        res[i].mv_x = qx;
+        res[i].mv_y = qy;
+        res[i].sad = best;
+        res[i].status = 0;
+        if (stats) {
+            stats[i].int_evals = sh.def_ie[di][d];
+            stats[i].q6_evals = Q.cnt.q6;
+            stats[i].qbil_evals = Q.cnt.qb;
+            stats[i].int_oob = sh.def_oob[di][d];
+        }
+    }
+}
+
+// Every lane adds 1, so a fetch advances the counter by 64 and the item index
+// is the first lane's old value / 64.  (A fetch under `if (lane == 0)`
+// followed by a broadcast lets the compiler run the other lanes through the
+// caller's loop on their own with a stale index; a uniform add compiles to one
+// LDS atomic, no branch.)
+__device__ __forceinline__ int fetch_item(int *counter)
+{
+    return __builtin_amdgcn_readfirstlane(atomicAdd(counter, 1)) >> 6;
 }
 
 template <int WAVES>
@@ -1049,6 +1104,7 @@ __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__res
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     if (wave == 0) {
+        // the chunk's valid calls, list order; invalid ones get their status
         int base = 0;
         for (int t0 = 0; t0 < cnt; t0 += 64) {
             const int t = t0 + lane;
@@ -1056,12 +1112,10 @@ __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__res
             tcme_call c{};
             if (t < cnt) {
                 c = calls[c0 + t];
-                if (c.blk == 8 || c.blk == 16) {
-                    PicDev R{}, C{};
-                    const int status = check_call(table, npics, c, R, C);
-                    if (status) put_status(res + c0 + t, stats ? stats + c0 + t : nullptr, status);
-                    else live = true;
-                }
+                PicDev R{}, C{};
+                const int status = check_call(table, npics, c, R, C);
+                if (status) put_status(res + c0 + t, stats ? stats + c0 + t : nullptr, status);
+                else live = true;
             }
             const uint64_t m = __ballot(live);
             if (live) {
@@ -1129,86 +1183,85 @@ __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__res
                 sh.lat[(2 * r + 1) * kLatPitch + c] = static_cast<uint32_t>(a >> 32);
             }
         } else {
-            lt.x1 = lt.x0;  // empty: every call takes the plane path
+            lt.x1 = lt.x0;  // empty: every call is deferred
         }
         if (wave == 0) {
-            // the run's calls by size, list order
-            int n16 = 0, n8 = 0;
+            // the run's calls by size class (64, 32, 16, 8), list order
+            int nc[4] = {0, 0, 0, 0};
             for (int p0 = pos; p0 < end; p0 += 64) {
                 const int p = p0 + lane;
-                const bool in = p < end;
-                const bool is16 = in && sh.call[p].blk == 16;
-                const bool is8 = in && !is16;
-                const uint64_t m16 = __ballot(is16), m8 = __ballot(is8);
+                const int k = p < end ? size_class(sh.call[p].blk) : 4;
                 const uint64_t below = (1ull << lane) - 1;
-                if (is16) sh.list[0][n16 + __popcll(m16 & below)] = static_cast<uint8_t>(p);
-                if (is8) sh.list[1][n8 + __popcll(m8 & below)] = static_cast<uint8_t>(p);
-                n16 += __popcll(m16);
-                n8 += __popcll(m8);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint64_t m = __ballot(k == q);
+                    if (k == q) sh.list[q][nc[q] + __popcll(m & below)] = static_cast<uint8_t>(p);
+                    nc[q] += __popcll(m);
+                }
             }
             if (lane == 0) {
-                sh.nlist[0] = n16;
-                sh.nlist[1] = n8;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) sh.nlist[q] = nc[q];
+                sh.ndef[0] = sh.ndef[1] = sh.ndef[2] = 0;
                 sh.next = 0;
-                sh.ndef = 0;
             }
         }
         __syncthreads();
-        const int n16 = sh.nlist[0], n8 = sh.nlist[1];
-        const int s16 = (n16 + 64 / kL16 - 1) / (64 / kL16);
-        const int s8 = (n8 + 64 / kL8 - 1) / (64 / kL8);
+        // slots, largest blocks first: 64 (1 call), 32 (4), 16 (8), 8 (16)
+        const int n64 = sh.nlist[0], n32 = sh.nlist[1], n16 = sh.nlist[2], n8 = sh.nlist[3];
+        const int s64 = n64, s32 = (n32 + 3) / 4, s16 = (n16 + 7) / 8, s8 = (n8 + 15) / 16;
         for (;;) {
-            // Every lane adds 1, so a fetch advances the counter by 64 and the
-            // slot index is the first lane's old value / 64.  (A fetch under
-            // `if (lane == 0)` followed by a broadcast lets the compiler run
-            // the other lanes through this loop on their own with a stale
-            // index; a uniform add compiles to one LDS atomic, no branch.)
-            const int s = __builtin_amdgcn_readfirstlane(atomicAdd(&sh.next, 1)) >> 6;
-            if (s >= s16 + s8) break;
-            if (s < s16) {
-                const int first = s * (64 / kL16);
-                run_slot<16, kL16>(sh, table, R, lw, lt, first, n16 - first, lane, res, stats);
-            } else {
-                const int first = (s - s16) * (64 / kL8);
-                run_slot<8, kL8>(sh, table, R, lw, lt, first, n8 - first, lane, res, stats);
+            int s = fetch_item(&sh.next);
+            if (s < s64) {
+                run_slot<64>(sh, table, R, lw, lt, s, n64 - s, lane, res, stats);
+                continue;
             }
+            s -= s64;
+            if (s < s32) {
+                run_slot<32>(sh, table, R, lw, lt, 4 * s, n32 - 4 * s, lane, res, stats);
+                continue;
+            }
+            s -= s32;
+            if (s < s16) {
+                run_slot<16>(sh, table, R, lw, lt, 8 * s, n16 - 8 * s, lane, res, stats);
+                continue;
+            }
+            s -= s16;
+            if (s < s8) {
+                run_slot<8>(sh, table, R, lw, lt, 16 * s, n8 - 16 * s, lane, res, stats);
+                continue;
+            }
+            break;
         }
         __syncthreads();
-        // deferred quarter-pel passes, the lattice LDS now reused as windows
-        const int ndef = sh.ndef;
-        if (ndef > 0) {
+        // Deferred quarter-pel passes; the lattice's LDS now holds anchor
+        // windows: (64+2)^2 for one 64x64 call, (32+2)^2 each for up to five
+        // 32x32 calls, (16+2)^2 for every wavefront's 8x8/16x16 call.
+        const int d64 = sh.ndef[0], d32 = sh.ndef[1], d16 = sh.ndef[2];
+        if (d64 + d32 + d16 > 0) {
+            constexpr int W32 = 34 * 34, W16 = 18 * 18;
+            constexpr int P32 = WAVES < 5 ? WAVES : 5;
+            // one 64x64 window fits: wavefront 0 takes them all
+            if (wave == 0)
+                for (int s = 0; s < d64; ++s) run_deferred<64>(sh, table, R, 0, s, sh.qw, lane, res, stats);
+            if (threadIdx.x == 0) sh.next = 0;
+            __syncthreads();
+            if (wave < P32) {
+                for (;;) {
+                    const int s = fetch_item(&sh.next);
+                    if (s >= d32) break;
+                    run_deferred<32>(sh, table, R, 1, s, sh.qw + wave * W32, lane, res, stats);
+                }
+            }
+            __syncthreads();
             if (threadIdx.x == 0) sh.next = 0;
             __syncthreads();
             for (;;) {
-                const int d = __builtin_amdgcn_readfirstlane(atomicAdd(&sh.next, 1)) >> 6;
-                if (d >= ndef) break;
-                const tcme_call c = sh.call[sh.def_ci[d]];
-                const PicDev C = table[c.cur_pic];
-                const int i = sh.idx[sh.def_ci[d]];
-                int32_t best = sh.def_sad[d];
-                int qx, qy;
-                Counts cnt;
-                if (c.blk == 16) {
-                    Searcher<16, 1> Q(R, C, c.px, c.py, &sh.qw[wave][0], nullptr);
-                    Q.qpel(sh.def_x[d], sh.def_y[d], qx, qy, best);
-                    cnt = Q.cnt;
-                } else {
-                    Searcher<8, 1> Q(R, C, c.px, c.py, &sh.qw[wave][0], nullptr);
-                    Q.qpel(sh.def_x[d], sh.def_y[d], qx, qy, best);
-                    cnt = Q.cnt;
-                }
-                if (lane == 0) {
-                    res[i].mv_x = qx;
-                    res[i].mv_y = qy;
-                    res[i].sad = best;
-                    res[i].status = 0;
-                    if (stats) {
-                        stats[i].int_evals = sh.def_ie[d];
-                        stats[i].q6_evals = cnt.q6;
-                        stats[i].qbil_evals = cnt.qb;
-                        stats[i].int_oob = sh.def_oob[d];
-                    }
-                }
+                const int s = fetch_item(&sh.next);
+                if (s >= d16) break;
+                const int blk = sh.call[sh.def_ci[2][s]].blk;
+                if (blk == 16) run_deferred<16>(sh, table, R, 2, s, sh.qw + wave * W16, lane, res, stats);
+                else run_deferred<8>(sh, table, R, 2, s, sh.qw + wave * W16, lane, res, stats);
             }
             __syncthreads();
         }

@@ -130,7 +130,7 @@ def test_gpu_rejects_bad_tiling(engine):
     with pytest.raises(me.TcmeError):
         engine.set_tiling(1, 1, 2, 4, 8, 0)
     with pytest.raises(me.TcmeError):
-        engine.set_tiling(1, 1, 2, 4, 4, 129)
+        engine.set_tiling(1, 1, 2, 4, 4, 97)
 
 
 def test_gpu_rejects_bad_descriptors(engine):
@@ -195,15 +195,15 @@ def test_dropin_symbol_matches_oracle(orc):
 
 
 TILINGS = [(1, 1, 1, 2, 0, 85), (1, 1, 2, 4, 4, 85), (1, 1, 4, 8, 8, 85), (1, 1, 1, 4, 8, 1),
-           (1, 1, 1, 4, 8, 37), (1, 1, 1, 4, 4, 128)]
+           (1, 1, 1, 4, 8, 37), (1, 1, 1, 4, 4, 96)]
 
 
 @pytest.mark.parametrize("tiling", TILINGS, ids=lambda t: "w%d_%d_%d_%d_t%d_c%d" % t)
 def test_gpu_tilings_agree_with_oracle(engine, orc, tiling):
-    """Every launch shape (wavefronts per 32x32 and 64x64 call; tile
-    workgroups of 0, 4 or 8 wavefronts and 1..128 calls for the 8x8/16x16
-    calls) gives the oracle's answers and candidate counts, for CTU-major,
-    reference-major and shuffled call orders."""
+    """Every launch shape (tile workgroups of 4 or 8 wavefronts and 1..96
+    calls; or one workgroup per call with 1..8 wavefronts by block size) gives
+    the oracle's answers and candidate counts, for CTU-major, reference-major
+    and shuffled call orders."""
     from tcodec_amd import synth, workload
     log(f"tiling {tiling}")
     fr = synth.scene_video(256, 200, 5, seed=9, rects=6, pan=(0.75, 0.25))
