@@ -52,6 +52,8 @@ def build_workload(eng):
 
 
 def kernel_name(tiling: dict) -> str:
+    if tiling["tile_waves"] and tiling["tile_big"]:
+        return "me_tile_kernel<4,big>"
     small = ("me_tile_kernel<%d>" % tiling["tile_waves"] if tiling["tile_waves"]
              else "me_search_kernel<8,1>+<16,1>")
     return ("%s+me_search_kernel<32,%d>+<64,%d> (concurrent)"
@@ -117,7 +119,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tiling", default=None,
-                    help="w8,w16,w32,w64[,tile_waves,tile_chunk]: wavefronts per call by "
+                    help="w8,w16,w32,w64[,tile_waves,tile_chunk[,tile_big]]: wavefronts per call by "
                          "block size and the tile workgroups for 8x8/16x16 calls; use this "
                          "launch shape instead of autotuning")
     ap.add_argument("--copy-mib", type=int, default=0,

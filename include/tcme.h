@@ -101,19 +101,23 @@ int  tcme_search(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *resu
                  tcme_stats *stats, void *stream);
 
 /* Launch shape of tcme_search.  Results never depend on it.
- *   tile_waves       4 or 8: tile workgroups of that many wavefronts each take
- *                    tile_chunk (1..96) consecutive calls, stage the reference
- *                    around each run of calls that share a reference picture
- *                    and a 64x64 centre tile in LDS, and search the run's
- *                    calls from there, many calls per wavefront (lists grouped
- *                    by CTU and reference, 85 calls per group, suit
- *                    tile_chunk 85);
- *                    0: every call gets its own workgroup of
- *   waves8..waves64  wavefronts, per block size (8x8: 1; 16x16: 1; 32x32: 1,
- *                    2 or 4; 64x64: 2, 4 or 8) */
+ *   waves8..waves64  calls searched one per workgroup: wavefronts per call
+ *                    by block size (8x8: 1; 16x16: 1; 32x32: 1, 2 or 4;
+ *                    64x64: 2, 4 or 8)
+ *   tile_waves       4 or 8: the 8x8 and 16x16 calls are searched by tile
+ *                    workgroups of that many wavefronts instead; each takes
+ *                    tile_chunk (1..96) consecutive calls, stages in LDS the
+ *                    reference around each run of calls that share a
+ *                    reference picture and a 64x64 centre tile, and searches
+ *                    many calls per wavefront from there.  Lists grouped by
+ *                    CTU and reference (85 calls per group) suit chunk 85.
+ *                    0: no tile workgroups.
+ *   tile_big         1 (with tile_waves 4): the tile workgroups take every
+ *                    call, all block sizes */
 typedef struct tcme_tiling {
     int32_t waves8, waves16, waves32, waves64;
     int32_t tile_waves, tile_chunk;
+    int32_t tile_big;
 } tcme_tiling;
 
 int  tcme_set_tiling(tcme_ctx *ctx, const tcme_tiling *t);

@@ -151,9 +151,10 @@ int pic_free(tcme_ctx *c, int id)
     return TCME_OK;
 }
 
-// Launches the search of n calls in launch shape t on stream s: the tile
-// kernel alone, or the four per-size kernels concurrently on the context's
-// streams, forked from and joined back into s with events.
+// Launches the search of n calls in launch shape t on stream s: the kernels
+// (the tile kernel for 8x8/16x16 or every size, per-size kernels for the
+// rest) run concurrently on the context's streams, forked from and joined
+// back into s with events.
 int dispatch(tcme_ctx *c, const tcme_call *calls, int n, const tcme::Tiling &t,
              tcme_result *res, tcme_stats *stats, hipStream_t s)
 {
@@ -161,17 +162,22 @@ int dispatch(tcme_ctx *c, const tcme_call *calls, int n, const tcme::Tiling &t,
     std::lock_guard<std::mutex> g(c->launch_mu);
     static const int sizes[4] = {64, 32, 16, 8};
     static const int slot[4] = {3, 2, 1, 0};
-    if (t.tile_waves) {
+    if (t.tile_waves && t.tile_big) {
         // one launch: the tile kernel takes every call
-        HIP_TRY(tcme::launch_tile(t.tile_waves, t.tile_chunk, c->d_table, c->max_pics, calls, n,
-                                  res, stats, s));
+        HIP_TRY(tcme::launch_tile(4, true, t.tile_chunk, c->d_table, c->max_pics, calls, n, res,
+                                  stats, s));
         return TCME_OK;
     }
     HIP_TRY(hipEventRecord(c->fork, s));
-    for (int k = 0; k < 4; ++k) {
+    const int nk = t.tile_waves ? 3 : 4;
+    for (int k = 0; k < nk; ++k) {
         HIP_TRY(hipStreamWaitEvent(c->sub[k], c->fork, 0));
-        HIP_TRY(tcme::launch_search_n(sizes[k], t.waves[slot[k]], c->d_table, c->max_pics, calls,
-                                      n, res, stats, c->sub[k]));
+        if (t.tile_waves && k == 2)
+            HIP_TRY(tcme::launch_tile(t.tile_waves, false, t.tile_chunk, c->d_table, c->max_pics,
+                                      calls, n, res, stats, c->sub[k]));
+        else
+            HIP_TRY(tcme::launch_search_n(sizes[k], t.waves[slot[k]], c->d_table, c->max_pics,
+                                          calls, n, res, stats, c->sub[k]));
         HIP_TRY(hipEventRecord(c->join[k], c->sub[k]));
         HIP_TRY(hipStreamWaitEvent(s, c->join[k], 0));
     }
@@ -322,6 +328,8 @@ int check_tiling(const tcme_tiling *t)
         return fail(TCME_E_ARG, "%d waves per tile workgroup not supported", t->tile_waves);
     if (t->tile_waves && (t->tile_chunk < 1 || t->tile_chunk > tcme::kTileChunkMax))
         return fail(TCME_E_ARG, "tile chunk %d outside 1..%d", t->tile_chunk, tcme::kTileChunkMax);
+    if (t->tile_big != 0 && !(t->tile_big == 1 && t->tile_waves == 4))
+        return fail(TCME_E_ARG, "tile_big needs tile_waves 4");
     return TCME_OK;
 }
 
@@ -334,6 +342,7 @@ tcme::Tiling to_internal(const tcme_tiling &t)
     r.waves[3] = t.waves64;
     r.tile_waves = t.tile_waves;
     r.tile_chunk = t.tile_waves ? t.tile_chunk : r.tile_chunk;
+    r.tile_big = t.tile_big;
     return r;
 }
 
@@ -356,11 +365,13 @@ TCME_API int tcme_get_tiling(tcme_ctx *c, tcme_tiling *t)
     t->waves64 = c->tiling.waves[3];
     t->tile_waves = c->tiling.tile_waves;
     t->tile_chunk = c->tiling.tile_chunk;
+    t->tile_big = c->tiling.tile_big;
     return TCME_OK;
 }
 
 // Times every launch shape (wavefronts per 32x32 and per 64x64 call, tile
-// workgroups of 0, 4 or 8 wavefronts; the tile chunk stays the context's) on
+// workgroups of 0, 4 or 8 wavefronts for the 8x8/16x16 calls or of 4 for
+// every call; the tile chunk stays the context's) on
 // the caller's call list (device arrays; results are overwritten with the
 // same answers) and keeps the fastest in the context.  Results do not depend
 // on the shape, only time does.
@@ -374,7 +385,7 @@ TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_resu
     hipStream_t s = static_cast<hipStream_t>(stream);
     static const int w32[] = {1, 2, 4};
     static const int w64[] = {2, 4, 8};
-    static const int tw[] = {0, 4, 8};
+    static const int tw[] = {0, 4, 8, -4};  // -4: tile_big with 4 wavefronts
     const int chunk = c->tiling.tile_chunk;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -387,7 +398,8 @@ TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_resu
                 tcme::Tiling t;
                 t.waves[2] = a;
                 t.waves[3] = b;
-                t.tile_waves = tl;
+                t.tile_waves = tl < 0 ? -tl : tl;
+                t.tile_big = tl < 0;
                 t.tile_chunk = chunk;
                 if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;  // warm
                 HIP_TRY(hipEventRecord(e0, s));

@@ -28,18 +28,22 @@ constexpr int kTileChunkMax = 96;
 // Launch shape of the search (chosen by tcme_autotune or set by the caller):
 // wavefronts per call for each block size (8x8 and 16x16: 1; 32x32: 1, 2
 // or 4; 64x64: 2, 4 or 8), and the tile workgroups that take the 8x8 and
-// 16x16 calls instead when tile_waves is 4 or 8.
+// 16x16 calls instead when tile_waves is 4 or 8 (and every call with
+// tile_big).
 struct Tiling {
     int waves[4] = {1, 1, 1, 4};  // for block sizes 8, 16, 32, 64
     int tile_waves = 8;
     int tile_chunk = 85;
+    int tile_big = 0;             // tile workgroups also take 32x32/64x64 (4 wavefronts)
 };
 
 bool valid_waves(int blk, int waves);
 // The tile kernel: tile workgroups of `waves` wavefronts, `chunk` calls each;
-// searches the calls of size 8 and 16 in the list.
-hipError_t launch_tile(int waves, int chunk, const PicDev *table, int npics, const tcme_call *calls,
-                       int n, tcme_result *res, tcme_stats *stats, hipStream_t s);
+// searches the calls of size 8 and 16 in the list, or, with `big` (4
+// wavefronts), every call.
+hipError_t launch_tile(int waves, bool big, int chunk, const PicDev *table, int npics,
+                       const tcme_call *calls, int n, tcme_result *res, tcme_stats *stats,
+                       hipStream_t s);
 hipError_t launch_anchors(const PicDev &p, hipStream_t s);
 // The search kernel for the calls of block size blk in the list (the others
 // are skipped); launch all four sizes to cover a mixed list.

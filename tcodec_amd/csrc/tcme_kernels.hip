@@ -1088,7 +1088,9 @@ __device__ __forceinline__ int fetch_item(int *counter)
     return __builtin_amdgcn_readfirstlane(atomicAdd(counter, 1)) >> 6;
 }
 
-template <int WAVES>
+// BIG: the workgroups also take the 32x32 and 64x64 calls (else those are
+// left to me_search_kernel, and calls of unsupported sizes to its 64 instance).
+template <int WAVES, bool BIG>
 __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__restrict__ table,
                                                             int npics,
                                                             const tcme_call *__restrict__ calls,
@@ -1112,10 +1114,12 @@ __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__res
             tcme_call c{};
             if (t < cnt) {
                 c = calls[c0 + t];
-                PicDev R{}, C{};
-                const int status = check_call(table, npics, c, R, C);
-                if (status) put_status(res + c0 + t, stats ? stats + c0 + t : nullptr, status);
-                else live = true;
+                if (BIG || c.blk == 8 || c.blk == 16) {
+                    PicDev R{}, C{};
+                    const int status = check_call(table, npics, c, R, C);
+                    if (status) put_status(res + c0 + t, stats ? stats + c0 + t : nullptr, status);
+                    else live = true;
+                }
             }
             const uint64_t m = __ballot(live);
             if (live) {
@@ -1212,16 +1216,18 @@ __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__res
         const int s64 = n64, s32 = (n32 + 3) / 4, s16 = (n16 + 7) / 8, s8 = (n8 + 15) / 16;
         for (;;) {
             int s = fetch_item(&sh.next);
-            if (s < s64) {
-                run_slot<64>(sh, table, R, lw, lt, s, n64 - s, lane, res, stats);
-                continue;
+            if constexpr (BIG) {
+                if (s < s64) {
+                    run_slot<64>(sh, table, R, lw, lt, s, n64 - s, lane, res, stats);
+                    continue;
+                }
+                s -= s64;
+                if (s < s32) {
+                    run_slot<32>(sh, table, R, lw, lt, 4 * s, n32 - 4 * s, lane, res, stats);
+                    continue;
+                }
+                s -= s32;
             }
-            s -= s64;
-            if (s < s32) {
-                run_slot<32>(sh, table, R, lw, lt, 4 * s, n32 - 4 * s, lane, res, stats);
-                continue;
-            }
-            s -= s32;
             if (s < s16) {
                 run_slot<16>(sh, table, R, lw, lt, 8 * s, n16 - 8 * s, lane, res, stats);
                 continue;
@@ -1241,19 +1247,22 @@ __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__res
         if (d64 + d32 + d16 > 0) {
             constexpr int W32 = 34 * 34, W16 = 18 * 18;
             constexpr int P32 = WAVES < 5 ? WAVES : 5;
-            // one 64x64 window fits: wavefront 0 takes them all
-            if (wave == 0)
-                for (int s = 0; s < d64; ++s) run_deferred<64>(sh, table, R, 0, s, sh.qw, lane, res, stats);
-            if (threadIdx.x == 0) sh.next = 0;
-            __syncthreads();
-            if (wave < P32) {
-                for (;;) {
-                    const int s = fetch_item(&sh.next);
-                    if (s >= d32) break;
-                    run_deferred<32>(sh, table, R, 1, s, sh.qw + wave * W32, lane, res, stats);
+            if constexpr (BIG) {
+                // one 64x64 window fits: wavefront 0 takes them all
+                if (wave == 0)
+                    for (int s = 0; s < d64; ++s)
+                        run_deferred<64>(sh, table, R, 0, s, sh.qw, lane, res, stats);
+                if (threadIdx.x == 0) sh.next = 0;
+                __syncthreads();
+                if (wave < P32) {
+                    for (;;) {
+                        const int s = fetch_item(&sh.next);
+                        if (s >= d32) break;
+                        run_deferred<32>(sh, table, R, 1, s, sh.qw + wave * W32, lane, res, stats);
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
             if (threadIdx.x == 0) sh.next = 0;
             __syncthreads();
             for (;;) {
@@ -1336,17 +1345,21 @@ bool valid_waves(int blk, int waves)
     }
 }
 
-hipError_t launch_tile(int waves, int chunk, const PicDev *table, int npics, const tcme_call *calls,
-                       int n, tcme_result *res, tcme_stats *stats, hipStream_t s)
+hipError_t launch_tile(int waves, bool big, int chunk, const PicDev *table, int npics,
+                       const tcme_call *calls, int n, tcme_result *res, tcme_stats *stats,
+                       hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     const int grid = (n + chunk - 1) / chunk;
-    if (waves == 4)
-        hipLaunchKernelGGL(me_tile_kernel<4>, dim3(grid), dim3(256), 0, s, table, npics, calls, n,
-                           chunk, res, stats);
+    if (big)
+        hipLaunchKernelGGL((me_tile_kernel<4, true>), dim3(grid), dim3(256), 0, s, table, npics,
+                           calls, n, chunk, res, stats);
+    else if (waves == 4)
+        hipLaunchKernelGGL((me_tile_kernel<4, false>), dim3(grid), dim3(256), 0, s, table, npics,
+                           calls, n, chunk, res, stats);
     else
-        hipLaunchKernelGGL(me_tile_kernel<8>, dim3(grid), dim3(512), 0, s, table, npics, calls, n,
-                           chunk, res, stats);
+        hipLaunchKernelGGL((me_tile_kernel<8, false>), dim3(grid), dim3(512), 0, s, table, npics,
+                           calls, n, chunk, res, stats);
     return hipGetLastError();
 }
 

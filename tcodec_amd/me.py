@@ -49,7 +49,8 @@ class Tiling(C.Structure):
     workgroups (wavefronts, calls per workgroup) for the 8x8/16x16 calls
     (tile_waves 0: one workgroup per call)."""
     _fields_ = [("waves8", C.c_int32), ("waves16", C.c_int32), ("waves32", C.c_int32),
-                ("waves64", C.c_int32), ("tile_waves", C.c_int32), ("tile_chunk", C.c_int32)]
+                ("waves64", C.c_int32), ("tile_waves", C.c_int32), ("tile_chunk", C.c_int32),
+                ("tile_big", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -209,8 +210,8 @@ class Engine:
         return t
 
     def set_tiling(self, waves8: int, waves16: int, waves32: int, waves64: int,
-                   tile_waves: int = 8, tile_chunk: int = 85) -> None:
-        t = Tiling(waves8, waves16, waves32, waves64, tile_waves, tile_chunk)
+                   tile_waves: int = 8, tile_chunk: int = 85, tile_big: int = 0) -> None:
+        t = Tiling(waves8, waves16, waves32, waves64, tile_waves, tile_chunk, tile_big)
         _check(self.L.tcme_set_tiling(self.h, C.byref(t)))
 
     def autotune(self, calls_ptr: int, n: int, res_ptr: int, reps: int = 3, stream: int = 0):

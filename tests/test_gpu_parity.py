@@ -131,6 +131,8 @@ def test_gpu_rejects_bad_tiling(engine):
         engine.set_tiling(1, 1, 2, 4, 8, 0)
     with pytest.raises(me.TcmeError):
         engine.set_tiling(1, 1, 2, 4, 4, 97)
+    with pytest.raises(me.TcmeError):
+        engine.set_tiling(1, 1, 2, 4, 8, 85, 1)
 
 
 def test_gpu_rejects_bad_descriptors(engine):
@@ -194,11 +196,12 @@ def test_dropin_symbol_matches_oracle(orc):
         assert (mx, my, sad) == orc.motion_est(new, cur, 0, 0, 40, 40, 16, 24)
 
 
-TILINGS = [(1, 1, 1, 2, 0, 85), (1, 1, 2, 4, 4, 85), (1, 1, 4, 8, 8, 85), (1, 1, 1, 4, 8, 1),
-           (1, 1, 1, 4, 8, 37), (1, 1, 1, 4, 4, 96)]
+TILINGS = [(1, 1, 1, 2, 0, 85, 0), (1, 1, 2, 4, 4, 85, 0), (1, 1, 4, 8, 8, 85, 0),
+           (1, 1, 1, 4, 8, 1, 0), (1, 1, 1, 4, 8, 37, 0), (1, 1, 1, 4, 4, 96, 0),
+           (1, 1, 1, 4, 4, 85, 1), (1, 1, 1, 4, 4, 7, 1)]
 
 
-@pytest.mark.parametrize("tiling", TILINGS, ids=lambda t: "w%d_%d_%d_%d_t%d_c%d" % t)
+@pytest.mark.parametrize("tiling", TILINGS, ids=lambda t: "w%d_%d_%d_%d_t%d_c%d_b%d" % t)
 def test_gpu_tilings_agree_with_oracle(engine, orc, tiling):
     """Every launch shape (tile workgroups of 4 or 8 wavefronts and 1..96
     calls; or one workgroup per call with 1..8 wavefronts by block size) gives
