@@ -25,6 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# 256 CUs x 64 lanes x 2.4 GHz: one 32-bit integer lane-op per lane per clock
+VALU_PEAK_TOPS = 39.3
 
 
 def log(msg: str) -> None:
@@ -51,26 +53,32 @@ def build_workload(eng):
     return scene, pics, frames
 
 
-def kernel_name(tiling: dict) -> str:
+def kernels(tiling: dict):
+    """[(kernel name, block sizes it searches)] in the library's launch order."""
     if tiling["tile_waves"] and tiling["tile_big"]:
-        return "me_tile_kernel<4,big>"
-    small = ("me_tile_kernel<%d>" % tiling["tile_waves"] if tiling["tile_waves"]
-             else "me_search_kernel<8,1>+<16,1>")
-    return ("%s+me_search_kernel<32,%d>+<64,%d> (concurrent)"
-            % (small, tiling["waves32"], tiling["waves64"]))
+        return [("me_tile_kernel<4,big>", (8, 16, 32, 64))]
+    out = [("me_search_kernel<64,%d>" % tiling["waves64"], (64,)),
+           ("me_search_kernel<32,%d>" % tiling["waves32"], (32,))]
+    if tiling["tile_waves"]:
+        out.append(("me_tile_kernel<%d>" % tiling["tile_waves"], (8, 16)))
+    else:
+        out += [("me_search_kernel<16,1>", (16,)), ("me_search_kernel<8,1>", (8,))]
+    return out
 
 
-def measured_traffic(tiling: dict):
-    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
-    committed PMC summary, when it was taken with this launch shape."""
-    path = os.path.join(ROOT, "profiles", "me_search_traffic.json")
+def measured_counters(kernel: str, tiling: dict):
+    """Per-launch PMC figures for `kernel` from the committed summary: HBM
+    bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) and VALU wave
+    instructions."""
+    path = os.path.join(ROOT, "profiles", "me_kernels_counters.json")
     try:
         t = json.load(open(path))
     except (OSError, ValueError):
         return None
-    if t.get("tiling") != tiling:
+    # a kernel's counters depend on its own shape (in its name) and the chunk
+    if t.get("tiling", {}).get("tile_chunk") != tiling["tile_chunk"]:
         return None
-    return int(t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"])
+    return t.get("kernels", {}).get(kernel)
 
 
 def cpu_baseline(scene, pics, frames, budget_s=12.0):
@@ -172,12 +180,26 @@ def main():
     # statistics pass (not timed): algorithmic bytes per frame
     step(with_stats=True)
     torch.cuda.synchronize()
-    algo = []
+    algo, per_call = [], []
     for i, (poc, kind, calls) in enumerate(frames):
         st = d_stats[i].cpu().numpy().view(me.STATS_DTYPE).ravel()
         algo.append(workload.algo_bytes(st, calls["blk"]))
+        per_call.append(st)
         assert (d_res[i][:, 3] == 0).all().item()
     log("statistics pass done")
+
+    def kernel_times(reps):
+        """Per-kernel durations (ms) per frame, events on each kernel's own
+        stream; the host waits after each frame, so this is a separate pass."""
+        eng.set_timing(True)
+        out = []
+        for _ in range(reps):
+            for i, (poc, kind, calls) in enumerate(frames):
+                eng.search_device(d_calls[i].data_ptr(), len(calls), d_res[i].data_ptr(), None,
+                                  sptr)
+                out.append(eng.last_timing())
+        eng.set_timing(False)
+        return np.array(out)
     def timed(label):
         for _ in range(args.warmup):
             step()
@@ -212,9 +234,21 @@ def main():
         elapsed, lat = plain_el, plain_lat
     nframes = len(frames) * args.steps
     fps = nframes / elapsed
-    kernel_s = lat.mean() / 1e3
-    achieved = float(np.mean(algo)) / kernel_s / 1e9
     calls_per_frame = int(np.mean([len(c) for _, _, c in frames]))
+    # the dominant (longest) kernel: its own calls' algorithmic bytes over its
+    # own duration, both per frame
+    tiling = best.as_dict()
+    ks = kernels(tiling)
+    kt = kernel_times(max(1, min(args.steps, 3))).mean(0)
+    dom = int(np.argmax(kt))
+    dom_name, dom_sizes = ks[dom]
+    dom_bytes = float(np.mean([workload.algo_bytes(st[np.isin(c["blk"], dom_sizes)],
+                                                   c["blk"][np.isin(c["blk"], dom_sizes)])
+                               for st, (_, _, c) in zip(per_call, frames)]))
+    dom_s = kt[dom] / 1e3
+    achieved = dom_bytes / dom_s / 1e9
+    counters = measured_counters(dom_name, tiling)
+    log("kernel ms per frame: " + ", ".join(f"{n} {t:.3f}" for (n, _), t in zip(ks, kt)))
 
     cpu = None if args.no_cpu else cpu_baseline(scene, pics, frames)
     out = {
@@ -242,10 +276,16 @@ def main():
         "tiling": best.as_dict(),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(best.as_dict()),
-                     "algo_bytes_per_frame": int(np.mean(algo)),
-                     "kernel": kernel_name(best.as_dict()),
-                     "kernel_ms_avg": round(kernel_s * 1e3, 3)},
+                     "traffic": counters["hbm_bytes"] if counters else None,
+                     "kernel": dom_name, "kernel_ms_avg": round(float(kt[dom]), 3),
+                     "kernel_algo_bytes": int(dom_bytes),
+                     "frame_algo_bytes": int(np.mean(algo)),
+                     "frame_effective_GBs": round(float(np.mean(algo)) / (lat.mean() / 1e3) / 1e9, 1),
+                     "kernels_ms": {n: round(float(t), 3) for (n, _), t in zip(ks, kt)}},
+        "valu": ({"int_ops_per_s": round(counters["valu_insts"] * 64 / dom_s / 1e12, 2),
+                  "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+                  "frac": round(counters["valu_insts"] * 64 / dom_s / 1e12 / VALU_PEAK_TOPS, 4),
+                  "kernel": dom_name} if counters else None),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

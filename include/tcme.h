@@ -129,6 +129,14 @@ int  tcme_get_tiling(tcme_ctx *ctx, tcme_tiling *t);
 int  tcme_autotune(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,
                    int reps, void *stream, tcme_tiling *best, float *best_ms);
 
+/* Per-kernel timing of dispatches (events on the stream each kernel runs
+ * on).  tcme_last_timing waits for the last dispatch and writes the duration
+ * in ms of each kernel it launched, in launch order: the 64x64 and 32x32
+ * per-call kernels, then the tile kernel (or the 16x16 and 8x8 per-call
+ * kernels); a single tile kernel with tile_big.  Returns the kernel count. */
+int  tcme_set_timing(tcme_ctx *ctx, int on);
+int  tcme_last_timing(tcme_ctx *ctx, float *ms, int max);
+
 /* Host-pointer convenience: copies calls in, searches, copies results out,
  * synchronises the stream. */
 int  tcme_search_host(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,

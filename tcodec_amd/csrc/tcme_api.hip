@@ -75,6 +75,11 @@ struct tcme_ctx {
     hipStream_t sub[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t fork = nullptr;
     hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
+    // optional per-kernel timing of the last dispatch (tcme_set_timing)
+    bool timing = false;
+    hipEvent_t t0[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t t1[4] = {nullptr, nullptr, nullptr, nullptr};
+    int nlaunched = 0;
     std::mutex mu;
     std::mutex launch_mu;
 };
@@ -164,23 +169,29 @@ int dispatch(tcme_ctx *c, const tcme_call *calls, int n, const tcme::Tiling &t,
     static const int slot[4] = {3, 2, 1, 0};
     if (t.tile_waves && t.tile_big) {
         // one launch: the tile kernel takes every call
+        if (c->timing) HIP_TRY(hipEventRecord(c->t0[0], s));
         HIP_TRY(tcme::launch_tile(4, true, t.tile_chunk, c->d_table, c->max_pics, calls, n, res,
                                   stats, s));
+        if (c->timing) HIP_TRY(hipEventRecord(c->t1[0], s));
+        c->nlaunched = 1;
         return TCME_OK;
     }
     HIP_TRY(hipEventRecord(c->fork, s));
     const int nk = t.tile_waves ? 3 : 4;
     for (int k = 0; k < nk; ++k) {
         HIP_TRY(hipStreamWaitEvent(c->sub[k], c->fork, 0));
+        if (c->timing) HIP_TRY(hipEventRecord(c->t0[k], c->sub[k]));
         if (t.tile_waves && k == 2)
             HIP_TRY(tcme::launch_tile(t.tile_waves, false, t.tile_chunk, c->d_table, c->max_pics,
                                       calls, n, res, stats, c->sub[k]));
         else
             HIP_TRY(tcme::launch_search_n(sizes[k], t.waves[slot[k]], c->d_table, c->max_pics,
                                           calls, n, res, stats, c->sub[k]));
+        if (c->timing) HIP_TRY(hipEventRecord(c->t1[k], c->sub[k]));
         HIP_TRY(hipEventRecord(c->join[k], c->sub[k]));
         HIP_TRY(hipStreamWaitEvent(s, c->join[k], 0));
     }
+    c->nlaunched = nk;
     return TCME_OK;
 }
 
@@ -221,7 +232,8 @@ TCME_API int tcme_create(tcme_ctx **out, int device, int max_pics)
     bool ok = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess;
     for (int k = 0; k < 4 && ok; ++k)
         ok = hipStreamCreateWithFlags(&c->sub[k], hipStreamNonBlocking) == hipSuccess &&
-             hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming) == hipSuccess;
+             hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreate(&c->t0[k]) == hipSuccess && hipEventCreate(&c->t1[k]) == hipSuccess;
     if (!ok) {
         tcme_destroy(c);
         return fail(TCME_E_HIP, "context streams/events");
@@ -244,6 +256,8 @@ TCME_API void tcme_destroy(tcme_ctx *c)
     for (int k = 0; k < 4; ++k) {
         if (c->sub[k]) hipStreamDestroy(c->sub[k]);
         if (c->join[k]) hipEventDestroy(c->join[k]);
+        if (c->t0[k]) hipEventDestroy(c->t0[k]);
+        if (c->t1[k]) hipEventDestroy(c->t1[k]);
     }
     if (c->fork) hipEventDestroy(c->fork);
     delete c;
@@ -418,6 +432,27 @@ TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_resu
     if (best_out) tcme_get_tiling(c, best_out);
     if (best_ms) *best_ms = best_t;
     return TCME_OK;
+}
+
+TCME_API int tcme_set_timing(tcme_ctx *c, int on)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    c->timing = on != 0;
+    c->nlaunched = 0;
+    return TCME_OK;
+}
+
+TCME_API int tcme_last_timing(tcme_ctx *c, float *ms, int max)
+{
+    if (!c || (max > 0 && !ms)) return fail(TCME_E_ARG, "null argument");
+    if (!c->timing) return fail(TCME_E_ARG, "timing is off");
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    const int n = c->nlaunched;
+    for (int k = 0; k < n && k < max; ++k) {
+        HIP_TRY(hipEventSynchronize(c->t1[k]));
+        HIP_TRY(hipEventElapsedTime(&ms[k], c->t0[k], c->t1[k]));
+    }
+    return n;
 }
 
 TCME_API int tcme_search_host(tcme_ctx *c, const tcme_call *calls, int n, tcme_result *results,

@@ -27,7 +27,8 @@ EXPORTS = (
     "tc_motion_est", "tcme_last_error", "tcme_version", "tcme_create", "tcme_destroy",
     "tcme_pic_create", "tcme_pic_destroy", "tcme_pic_upload", "tcme_pic_upload_device",
     "tcme_pic_luma", "tcme_pic_finish", "tcme_search", "tcme_search_host",
-    "tcme_set_tiling", "tcme_get_tiling", "tcme_autotune",
+    "tcme_set_tiling", "tcme_get_tiling", "tcme_autotune", "tcme_set_timing",
+    "tcme_last_timing",
 )
 
 CALL_DTYPE = np.dtype([("ref_pic", "<i4"), ("cur_pic", "<i4"), ("px", "<i4"), ("py", "<i4"),
@@ -114,6 +115,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tcme_get_tiling.argtypes = [C.c_void_p, C.POINTER(Tiling)]
     L.tcme_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                 C.c_void_p, C.POINTER(Tiling), C.POINTER(C.c_float)]
+    L.tcme_set_timing.argtypes = [C.c_void_p, C.c_int]
+    L.tcme_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int]
     _lib = L
     return L
 
@@ -222,6 +225,15 @@ class Engine:
         _check(self.L.tcme_autotune(self.h, C.c_void_p(calls_ptr), n, C.c_void_p(res_ptr), reps,
                                     C.c_void_p(stream), C.byref(t), C.byref(ms)))
         return t, ms.value
+
+    def set_timing(self, on: bool) -> None:
+        _check(self.L.tcme_set_timing(self.h, int(on)))
+
+    def last_timing(self) -> list[float]:
+        """Per-kernel ms of the last dispatch (waits for it), in launch order."""
+        ms = (C.c_float * 4)()
+        n = _check(self.L.tcme_last_timing(self.h, ms, 4))
+        return [ms[k] for k in range(n)]
 
     def search_device(self, calls_ptr: int, n: int, res_ptr: int, stats_ptr: int | None = None,
                       stream: int = 0) -> None:

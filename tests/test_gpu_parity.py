@@ -264,3 +264,25 @@ def test_gpu_autotune_keeps_results(engine, orc):
         engine.set_tiling(**old.as_dict())
         engine.free(rp)
         engine.free(cp)
+
+
+def test_gpu_kernel_timing(engine):
+    """Per-kernel timing: one duration per launched kernel, in launch order."""
+    from tcodec_amd import synth, workload
+    fr = synth.scene_video(256, 192, 2, seed=4, rects=4)
+    rp, cp = engine.pic(fr[0][0]), engine.pic(fr[1][0])
+    calls = workload.frame_calls_by_ctu(256, 192, [rp], cp, workload.SR_P)
+    old = engine.tiling()
+    try:
+        engine.set_timing(True)
+        for tiling, count in (((1, 1, 1, 4, 8, 85, 0), 3), ((1, 1, 1, 4, 4, 85, 1), 1),
+                              ((1, 1, 1, 4, 0, 85, 0), 4)):
+            engine.set_tiling(*tiling)
+            engine.search(calls)
+            ms = engine.last_timing()
+            assert len(ms) == count and all(t > 0 for t in ms), (tiling, ms)
+    finally:
+        engine.set_timing(False)
+        engine.set_tiling(**old.as_dict())
+        engine.free(rp)
+        engine.free(cp)
