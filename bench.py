@@ -120,6 +120,115 @@ def cpu_baseline(scene, pics, frames, budget_s=12.0):
                       f"{dt:.1f} s, 1 thread"}
 
 
+def batched_form(eng, steps: int):
+    """64 independent P-only streams of uneven size (16 x 320x240, 24 x
+    1280x720, 16 x 1920x1080, 8 x 3840x2160), 4 references, gradient source
+    with a seed per stream: one launch per time step over every stream's next
+    picture, against one launch per picture (the plain per-frame form)."""
+    import torch
+    from tcodec_amd import serve
+    nf = steps + 5                        # 1 intra + warm-up to 4 references + timed
+    mix = [("240p", 16), ("720p", 24), ("1080p", 16), ("2160p", 8)]
+
+    def streams():
+        out, sid = [], 0
+        for name, count in mix:
+            w, h = serve.RES[name]
+            for _ in range(count):
+                out.append(serve.Stream(sid, w, h, nf, "P", clips[sid]))
+                sid += 1
+        return out
+
+    clips, sid = {}, 0
+    for name, count in mix:
+        w, h = serve.RES[name]
+        for _ in range(count):
+            clips[sid] = serve.DeviceGradientClip(w, h, nf, seed=1000 + sid)
+            sid += 1
+    out = {}
+    for form, per_stream in (("batched", False), ("plain", True)):
+        srv = serve.Server(eng, hbm_budget=1 << 62, max_tick_calls=1 << 62,
+                           per_stream_launch=per_stream)
+        for st in streams():
+            srv.submit(st)
+        srv._admit(time.perf_counter())
+        for _ in range(5):                # intra + first four P pictures
+            srv.tick(time.perf_counter())
+        torch.cuda.synchronize()
+        f0, c0, t0 = srv.frames, srv.calls, time.perf_counter()
+        while srv.active:
+            srv.tick(time.perf_counter())
+        dt = time.perf_counter() - t0
+        out[form] = {"frames_per_s": round((srv.frames - f0) / dt, 1),
+                     "calls_per_s": round((srv.calls - c0) / dt),
+                     "ms_per_step": round(dt * 1e3 / steps, 2)}
+        log(f"{form}: {out[form]}")
+    clips.clear()
+    torch.cuda.empty_cache()
+    return {"streams": 64, "mix": {n: c for n, c in mix}, "refs": 4, "timed_steps": steps,
+            "source": "diagonal gradient + uniform noise [-4,4], (+3,-2) px/frame, per-stream seed",
+            "launch_per_step": out["batched"], "launch_per_picture": out["plain"],
+            "speedup": round(out["batched"]["frames_per_s"] / out["plain"]["frames_per_s"], 2)}
+
+
+def served_form(eng, rate: float, duration: float, budget_gib: float):
+    """Poisson arrivals of B-GOP4 4-reference encode requests (30% 320x240,
+    40% 1280x720, 25% 1920x1080, 5% 3840x2160; 30..120 pictures each) served
+    with continuous batching under an HBM budget."""
+    import torch
+    from tcodec_amd import serve
+    clips = {name: serve.Clip(w, h, 3 if w > 2000 else 5, seed=77 + i)
+             for i, (name, (w, h)) in enumerate(serve.RES.items())}
+    reqs = serve.poisson_requests(rate, duration, seed=5, clips=clips)
+    srv = serve.Server(eng, hbm_budget=int(budget_gib * (1 << 30)))
+    for r in reqs:
+        srv.submit(r)
+    torch.cuda.synchronize()
+    wall = srv.run()
+    lat = srv.latencies()
+    lat1080 = np.concatenate([np.array(s.latencies) for s in srv.finished
+                              if (s.w, s.h) == (1920, 1080)] or [np.zeros(0)])
+    out = {"rate_per_s": rate, "arrivals_s": duration, "requests": len(reqs),
+           "frames": srv.frames, "wall_s": round(wall, 2),
+           "frames_per_s": round(srv.frames / wall, 1),
+           "equiv_1080p_frames_per_s": round(srv.calls / wall / 171600, 1),
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                          "p99": round(float(np.percentile(lat, 99)), 2)},
+           "latency_1080p_ms": ({"p50": round(float(np.percentile(lat1080, 50)), 2),
+                                 "p99": round(float(np.percentile(lat1080, 99)), 2)}
+                                if lat1080.size else None),
+           "ticks": srv.ticks, "hbm_budget_GiB": budget_gib,
+           "source": "moving-rectangle scene clips per resolution (5 pictures; 3 at 3840x2160)"}
+    log(f"served: {out}")
+    del clips
+    torch.cuda.empty_cache()
+    return out
+
+
+def dropin_rate(frames, scene, pics, samples: int = 400):
+    """The per-call drop-in symbol on a sample of one frame's calls: one
+    launch per call.  Returns ME frames/s."""
+    from tcodec_amd import me
+    poc, kind, calls = frames[0]
+    poc_of = {pid: t for t, pid in pics.items()}
+    cur = scene.frame(poc_of[int(calls["cur_pic"][0])])[0]
+    refs = {}
+    rng = np.random.default_rng(3)
+    idx = rng.choice(len(calls), samples, replace=False)
+    me.motion_est(np.zeros((64, 64), np.uint8), 64, 64, 64, np.zeros((8, 8), np.uint8), 0, 8,
+                  0, 0, 8, 8)  # context set-up outside the timing
+    t0 = time.perf_counter()
+    for i in idx:
+        c = calls[i]
+        rp = int(c["ref_pic"])
+        if rp not in refs:
+            refs[rp] = scene.frame(poc_of[rp])[0]
+        me.motion_est(refs[rp], 1920, 1920, 1080, cur, int(c["py"]) * 1920 + int(c["px"]), 1920,
+                      int(c["cx"]), int(c["cy"]), int(c["blk"]), int(c["sr"]))
+    dt = time.perf_counter() - t0
+    return samples / dt / len(calls)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +241,13 @@ def main():
                          "launch shape instead of autotuning")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (counter calibration)")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the batched (64 streams) and served (Poisson) runs")
+    ap.add_argument("--batch-steps", type=int, default=4)
+    ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-seconds", type=float, default=2.0,
+                    help="arrival window of the served run (the serving config uses 120)")
+    ap.add_argument("--serve-budget-gib", type=float, default=48.0)
     ap.add_argument("--mode", choices=("tuned", "default"), default="tuned",
                     help="default: the built-in launch shape only; tuned: also the autotuned "
                          "shape (reported as value)")
@@ -251,6 +367,10 @@ def main():
     log("kernel ms per frame: " + ", ".join(f"{n} {t:.3f}" for (n, _), t in zip(ks, kt)))
 
     cpu = None if args.no_cpu else cpu_baseline(scene, pics, frames)
+    dropin = None if args.no_forms else dropin_rate(frames, scene, pics)
+    batched = None if args.no_forms else batched_form(eng, args.batch_steps)
+    served = None if args.no_forms else served_form(eng, args.serve_rate, args.serve_seconds,
+                                                    args.serve_budget_gib)
     out = {
         "metric": "ME frames/s @1080p 4-ref + p50/p99 frame latency (mixed load), bit-exact",
         "value": round(fps, 2),
@@ -287,6 +407,11 @@ def main():
                   "frac": round(counters["valu_insts"] * 64 / dom_s / 1e12 / VALU_PEAK_TOPS, 4),
                   "kernel": dom_name} if counters else None),
         "cpu_baseline": cpu,
+        "dropin_per_call": ({"frames_per_s": round(dropin, 4),
+                             "note": "tc_motion_est, one launch per call, host planes"}
+                            if dropin else None),
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
     eng.close()

@@ -1,0 +1,315 @@
+"""Batched and served forms of the search: many encode streams, one launch per tick.
+
+A tick takes the next coded picture of every active stream, and searches all
+their calls (every in-frame quadtree node against the picture's references, as
+in ``workload.frame_calls_by_ctu``) in ONE ``tcme_search`` launch.  Streams
+keep their reference pictures resident on the device: each stream owns a ring
+of 5 device pictures (4 references + the current one, the reference encoder's
+4-slot picture buffer, src/encoder.c:3085-3097) allocated when it is admitted
+and freed when it finishes; a new picture is copied into the ring slot of the
+one leaving the buffer (device to device, anchors rebuilt).
+
+``Server`` is the serving loop: requests arrive over time (Poisson), wait in a
+queue, and are admitted while their pictures fit an HBM budget and the tick's
+call count stays under a work bound; admitted streams join the very next tick
+(continuous batching, no draining).  A frame is ready when its stream's
+previous frame is done (the encoder's dependency on reconstructed references);
+its latency runs from ready to the end of the tick that searched it.
+
+Source pictures come from device-resident clips per resolution (``Clip``), so
+the timed loop does no host-side generation.  Results are pure functions of
+(pictures, calls), so a stream's results are the same alone and in any mix:
+``run_alone`` and tests/test_gpu_serve.py check exactly that.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import me, synth, workload
+
+RES = {"240p": (320, 240), "720p": (1280, 720), "1080p": (1920, 1080), "2160p": (3840, 2160)}
+
+
+def pic_bytes(w: int, h: int) -> int:
+    """Device bytes of one reference picture: luma + anchors (8 B per sample)
+    + tail slack, rows padded to 64."""
+    pitch = (w + 63) // 64 * 64
+    return pitch * h * 9 + 512
+
+
+def stream_bytes(w: int, h: int) -> int:
+    return 5 * pic_bytes(w, h)
+
+
+class Clip:
+    """F source luma planes of one resolution, resident on the device."""
+
+    def __init__(self, w: int, h: int, frames: int, seed: int, kind: str = "scene"):
+        self.w, self.h, self.frames = w, h, frames
+        if kind == "scene":
+            sc = synth.Scene(w, h, seed=seed, rects=48 if w >= 1920 else 16, pan=(0.75, 0.25))
+            planes = [sc.frame(t)[0] for t in range(frames)]
+        else:
+            planes = [f[0] for f in synth.gradient_video(w, h, frames, seed=seed)]
+        self.host = np.stack(planes)
+        self.dev = torch.from_numpy(self.host).cuda()
+
+    def plane_ptr(self, t: int) -> int:
+        return self.dev[t % self.frames].data_ptr()
+
+
+class DeviceGradientClip(Clip):
+    """The diagonal-gradient source (translated (+3,-2) px per frame, uniform
+    integer noise in [-4,4]) rendered on the device with the device's own
+    counter RNG seeded per stream: many streams of large pictures without
+    host-side generation.  Same structure as synth.gradient_video; the noise
+    values differ from the host generator's."""
+
+    def __init__(self, w: int, h: int, frames: int, seed: int):
+        self.w, self.h, self.frames = w, h, frames
+        dev = torch.device("cuda")
+        yy = torch.arange(h, device=dev, dtype=torch.int64).view(-1, 1)
+        xx = torch.arange(w, device=dev, dtype=torch.int64).view(1, -1)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        out = torch.empty((frames, h, w), dtype=torch.uint8, device=dev)
+        for t in range(frames):
+            g = ((xx - 3 * t) + (yy + 2 * t)) * 255 // (w + h - 2)
+            nz = torch.randint(-4, 5, (h, w), generator=gen, device=dev)
+            out[t] = (g + nz).clamp_(0, 255).to(torch.uint8)
+        self.dev = out
+        self.host = None
+
+
+class Templates:
+    """Device call-list templates per (w, h, refs, sr): picture ids are slots
+    0..refs-1 for the references and `refs` for the current picture."""
+
+    def __init__(self):
+        self.cache: dict[tuple, torch.Tensor] = {}
+
+    def get(self, w, h, refs, sr) -> torch.Tensor:
+        key = (w, h, refs, sr)
+        t = self.cache.get(key)
+        if t is None:
+            calls = workload.frame_calls_by_ctu(w, h, list(range(refs)), refs, sr)
+            t = torch.from_numpy(calls.view(np.int32).reshape(-1, 8).copy()).cuda()
+            self.cache[key] = t
+        return t
+
+
+@dataclass
+class Stream:
+    sid: int
+    w: int
+    h: int
+    nframes: int
+    gop: str                      # "P" or "B" (GOP4 ladder)
+    clip: Clip
+    offset: int = 0
+    arrival: float = 0.0
+    order: list = field(default_factory=list)
+    pos: int = 0                  # next index in coding order
+    ring: list = field(default_factory=list)   # device picture ids
+    slot_of: dict = field(default_factory=dict)  # poc -> ring index
+    ready_at: float = 0.0
+    latencies: list = field(default_factory=list)
+    results: list = field(default_factory=list)
+
+    def __post_init__(self):
+        if self.gop == "B":
+            n = 4 * max(1, (self.nframes - 1 + 3) // 4) + 1
+            self.order = workload.gop4_coding_order(n)
+        else:
+            self.order = workload.p_coding_order(self.nframes)
+
+    @property
+    def done(self) -> bool:
+        return self.pos >= len(self.order)
+
+    def activate(self, eng: me.Engine) -> None:
+        self.ring = [eng.new_pic(self.w, self.h) for _ in range(5)]
+        self.slot_of = {}
+
+    def release(self, eng: me.Engine) -> None:
+        for p in self.ring:
+            eng.free(p)
+        self.ring = []
+
+    def prepare(self, eng: me.Engine, tpl: Templates, stream_ptr: int):
+        """Loads the next coded picture into the ring and returns its call
+        list on the device (None for an intra picture).  Advances pos."""
+        poc, kind, dpb = self.order[self.pos]
+        self.pos += 1
+        keep = set(dpb)
+        slot = next(i for i in range(5) if i not in
+                    {s for p, s in self.slot_of.items() if p in keep})
+        self.slot_of = {p: s for p, s in self.slot_of.items() if p in keep}
+        self.slot_of[poc] = slot
+        eng.upload_device(self.ring[slot], self.clip.plane_ptr(self.offset + poc), self.w,
+                          stream_ptr)
+        if kind == "I" or not dpb:
+            return None
+        sr = workload.SR_B if kind == "B" else workload.SR_P
+        t = tpl.get(self.w, self.h, len(dpb), sr)
+        lut = torch.tensor([self.ring[self.slot_of[p]] for p in dpb] + [self.ring[slot]],
+                           dtype=torch.int32).cuda(non_blocking=True)
+        calls = t.clone()
+        calls[:, 0] = lut[t[:, 0].long()]
+        calls[:, 1] = lut[len(dpb)]
+        return calls
+
+
+class Server:
+    """Continuous-batching request loop over one tcme context."""
+
+    def __init__(self, eng: me.Engine, hbm_budget: int, max_tick_calls: int = 6_000_000,
+                 keep_results: bool = False, per_stream_launch: bool = False):
+        self.eng = eng
+        self.per_stream = per_stream_launch
+        self.budget = hbm_budget
+        self.max_tick_calls = max_tick_calls
+        self.keep = keep_results
+        self.tpl = Templates()
+        self.stream = torch.cuda.current_stream()
+        self.sptr = self.stream.cuda_stream
+        self.queue: list[Stream] = []
+        self.active: list[Stream] = []
+        self.finished: list[Stream] = []
+        self.used = 0
+        self.frames = 0           # searched (non-intra) pictures
+        self.calls = 0
+        self.ticks = 0
+        self.tick_ms: list[float] = []
+
+    def submit(self, s: Stream) -> None:
+        self.queue.append(s)
+
+    _nodes: dict = {}
+
+    @classmethod
+    def calls_per_frame(cls, s: Stream) -> int:
+        """Upper bound (4 references) of a picture's calls."""
+        n = cls._nodes.get((s.w, s.h))
+        if n is None:
+            n = cls._nodes[(s.w, s.h)] = len(workload.qt_nodes(s.w, s.h))
+        return 4 * n
+
+    def _admit(self, now: float) -> None:
+        work = sum(self.calls_per_frame(s) for s in self.active)
+        while self.queue and self.queue[0].arrival <= now:
+            s = self.queue[0]
+            need = stream_bytes(s.w, s.h)
+            cost = self.calls_per_frame(s)
+            if self.used + need > self.budget or (self.active and work + cost > self.max_tick_calls):
+                break
+            self.queue.pop(0)
+            s.activate(self.eng)
+            s.ready_at = now
+            self.used += need
+            work += cost
+            self.active.append(s)
+
+    def tick(self, now: float) -> None:
+        """One launch over the next picture of every active stream."""
+        lists, owners = [], []
+        for s in self.active:
+            c = s.prepare(self.eng, self.tpl, self.sptr)
+            if c is not None:
+                lists.append(c)
+                owners.append((s, c.shape[0]))
+        if lists and self.per_stream:
+            # the plain form: one launch per picture, back to back
+            outs = []
+            for c in lists:
+                r = torch.empty((c.shape[0], 4), dtype=torch.int32, device="cuda")
+                self.eng.search_device(c.data_ptr(), c.shape[0], r.data_ptr(), None, self.sptr)
+                outs.append(r)
+            res = torch.cat(outs) if self.keep else None
+        elif lists:
+            calls = torch.cat(lists) if len(lists) > 1 else lists[0]
+            res = torch.empty((calls.shape[0], 4), dtype=torch.int32, device="cuda")
+            self.eng.search_device(calls.data_ptr(), calls.shape[0], res.data_ptr(), None,
+                                   self.sptr)
+        self.stream.synchronize()
+        end = time.perf_counter()
+        if lists and self.keep:
+            host = res.cpu().numpy()
+            o = 0
+            for s, n in owners:
+                s.results.append(host[o:o + n].copy())
+                o += n
+        self.ticks += 1
+        self.tick_ms.append((end - now) * 1e3)
+        searched = {id(s): n for s, n in owners}
+        still = []
+        for s in self.active:
+            if id(s) in searched:
+                s.latencies.append((end - s.ready_at) * 1e3)
+                self.frames += 1
+                self.calls += searched[id(s)]
+            s.ready_at = end
+            if s.done:
+                s.release(self.eng)
+                self.used -= stream_bytes(s.w, s.h)
+                self.finished.append(s)
+            else:
+                still.append(s)
+        self.active = still
+
+    def run(self, t0: float | None = None) -> float:
+        """Serves until every submitted request is finished; arrival times are
+        seconds after t0.  Returns the wall time used."""
+        self.queue.sort(key=lambda s: s.arrival)
+        t0 = time.perf_counter() if t0 is None else t0
+        for s in self.queue:
+            s.arrival += t0
+        while self.queue or self.active:
+            now = time.perf_counter()
+            self._admit(now)
+            if not self.active:
+                wait = self.queue[0].arrival - now
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+                continue
+            self.tick(now)
+        return time.perf_counter() - t0
+
+    def latencies(self) -> np.ndarray:
+        return np.concatenate([np.array(s.latencies) for s in self.finished]) if self.finished \
+            else np.zeros(0)
+
+
+def poisson_requests(rate: float, duration: float, seed: int, clips: dict,
+                     mix=(("240p", 0.30), ("720p", 0.40), ("1080p", 0.25), ("2160p", 0.05)),
+                     frames=(30, 120), gop: str = "B") -> list[Stream]:
+    """Requests of the serving config: Poisson arrivals, resolution mix,
+    30..120 frames each (uniform), GOP4 B, sources from `clips`."""
+    rng = np.random.default_rng(seed)
+    out, t, sid = [], 0.0, 0
+    names = [m[0] for m in mix]
+    probs = np.array([m[1] for m in mix])
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        name = names[int(rng.choice(len(names), p=probs / probs.sum()))]
+        w, h = RES[name]
+        nf = int(rng.integers(frames[0], frames[1] + 1))
+        out.append(Stream(sid, w, h, nf, gop, clips[name], offset=int(rng.integers(0, 1000)),
+                          arrival=t))
+        sid += 1
+    return out
+
+
+def run_alone(eng: me.Engine, s: Stream) -> list[np.ndarray]:
+    """The same stream served by itself: its per-frame results."""
+    solo = Stream(s.sid, s.w, s.h, s.nframes, s.gop, s.clip, s.offset)
+    srv = Server(eng, hbm_budget=1 << 62, keep_results=True)
+    srv.submit(solo)
+    srv.run()
+    return solo.results
