@@ -229,6 +229,43 @@ def dropin_rate(frames, scene, pics, samples: int = 400):
     return samples / dt / len(calls)
 
 
+def mc_leg(eng, frames, d_res, reps: int = 5):
+    """Luma motion compensation of every block of the first frame at its own
+    search result (the consumer of each search): one launch, blocks packed."""
+    import torch
+    from tcodec_amd import me
+    poc, kind, calls = frames[0]
+    res = d_res[0].cpu().numpy()
+    n = calls["blk"].astype(np.int64)
+    preds = np.zeros(len(calls), me.PRED_DTYPE)
+    preds["pic"] = calls["ref_pic"]
+    preds["mv_x"], preds["mv_y"] = res[:, 0], res[:, 1]
+    preds["blk"] = calls["blk"]
+    preds["out_stride"] = calls["blk"]
+    preds["out_off"] = np.concatenate([[0], np.cumsum(n * n)[:-1]])
+    nbytes = int((n * n).sum())
+    d_p = torch.from_numpy(preds.view(np.uint8).copy()).cuda()
+    d_o = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    s = torch.cuda.current_stream()
+    eng.predict_device(d_p.data_ptr(), len(preds), d_o.data_ptr(), nbytes, None, False,
+                       s.cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        eng.predict_device(d_p.data_ptr(), len(preds), d_o.data_ptr(), nbytes, None, False,
+                           s.cuda_stream)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    algo = int(((n + 5) ** 2 + n * n).sum())   # reference apron read + block written
+    out = {"kernel": "mc_luma_kernel", "blocks": len(preds), "pixels": nbytes,
+           "ms": round(ms, 3), "Gpixels_per_s": round(nbytes / ms / 1e6, 2),
+           "algo_bytes": algo, "achieved_GBs": round(algo / ms / 1e6, 1),
+           "frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    log(f"mc: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -367,6 +404,7 @@ def main():
     log("kernel ms per frame: " + ", ".join(f"{n} {t:.3f}" for (n, _), t in zip(ks, kt)))
 
     cpu = None if args.no_cpu else cpu_baseline(scene, pics, frames)
+    mc = mc_leg(eng, frames, d_res)
     dropin = None if args.no_forms else dropin_rate(frames, scene, pics)
     batched = None if args.no_forms else batched_form(eng, args.batch_steps)
     served = None if args.no_forms else served_form(eng, args.serve_rate, args.serve_seconds,
@@ -410,6 +448,7 @@ def main():
         "dropin_per_call": ({"frames_per_s": round(dropin, 4),
                              "note": "tc_motion_est, one launch per call, host planes"}
                             if dropin else None),
+        "mc": mc,
         "batched": batched,
         "served": served,
     }

@@ -84,6 +84,9 @@ void tcme_destroy(tcme_ctx *ctx);
 
 /* Allocates a w x h luma picture with its half-sample planes on the device. */
 int  tcme_pic_create(tcme_ctx *ctx, int w, int h);
+/* A plain w x h plane with no anchor plane: current pictures and chroma
+ * planes (it cannot be a search or luma-prediction reference). */
+int  tcme_pic_create_plane(tcme_ctx *ctx, int w, int h);
 int  tcme_pic_destroy(tcme_ctx *ctx, int pic);
 /* Fills a picture from host (upload) or device memory, then derives its raw
  * 6-tap half-sample planes.  Asynchronous on stream (a hipStream_t). */
@@ -128,6 +131,31 @@ int  tcme_get_tiling(tcme_ctx *ctx, tcme_tiling *t);
  * the context.  Synchronises the stream. */
 int  tcme_autotune(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,
                    int reps, void *stream, tcme_tiling *best, float *best_ms);
+
+/* ---- motion compensation (batched) -------------------------------------- */
+
+/* One predicted block: picture id, absolute motion vector (quarter sample
+ * for luma, eighth sample on a chroma plane), block size (1..64), and where
+ * the block goes in the caller's output buffer (byte offset of its first
+ * sample, row stride). */
+typedef struct tcme_pred {
+    int32_t pic, mv_x, mv_y, blk;
+    int64_t out_off;
+    int32_t out_stride, reserved;
+} tcme_pred;
+
+/* Replaces tc_inter_predict (reference src/motion.c:493-584, prototype
+ * include/tcodec_common.h:464-468) for n blocks at once: pic must be a
+ * reference picture (tcme_pic_create).  preds, out and status (optional, one
+ * TCME_* code per block) are device arrays; out holds out_bytes bytes.
+ * Asynchronous on stream. */
+int  tcme_predict_luma(tcme_ctx *ctx, const tcme_pred *preds, int n, uint8_t *out,
+                       int64_t out_bytes, int32_t *status, void *stream);
+/* Replaces tc_inter_predict_chroma (reference src/motion.c:597-632,
+ * include/tcodec_common.h:476-480) for n blocks: pic is a chroma plane
+ * (tcme_pic_create_plane of the chroma size). */
+int  tcme_predict_chroma(tcme_ctx *ctx, const tcme_pred *preds, int n, uint8_t *out,
+                         int64_t out_bytes, int32_t *status, void *stream);
 
 /* Per-kernel timing of dispatches (events on the stream each kernel runs
  * on).  tcme_last_timing waits for the last dispatch and writes the duration

@@ -271,6 +271,14 @@ TCME_API int tcme_pic_create(tcme_ctx *c, int w, int h)
     return pic_alloc(c, w, h, true);
 }
 
+TCME_API int tcme_pic_create_plane(tcme_ctx *c, int w, int h)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    return pic_alloc(c, w, h, false);
+}
+
 TCME_API int tcme_pic_destroy(tcme_ctx *c, int pic)
 {
     if (int e = check_pic(c, pic)) return e;
@@ -432,6 +440,35 @@ TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_resu
     if (best_out) tcme_get_tiling(c, best_out);
     if (best_ms) *best_ms = best_t;
     return TCME_OK;
+}
+
+namespace {
+
+int predict(tcme_ctx *c, bool chroma, const tcme_pred *preds, int n, uint8_t *out,
+            int64_t out_bytes, int32_t *status, void *stream)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    if (n < 0 || (n > 0 && (!preds || !out || out_bytes <= 0)))
+        return fail(TCME_E_ARG, "bad prediction list");
+    if (n == 0) return TCME_OK;
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    HIP_TRY(tcme::launch_predict(chroma, c->d_table, c->max_pics, preds, n, out, out_bytes, status,
+                                 static_cast<hipStream_t>(stream)));
+    return TCME_OK;
+}
+
+}  // namespace
+
+TCME_API int tcme_predict_luma(tcme_ctx *c, const tcme_pred *preds, int n, uint8_t *out,
+                               int64_t out_bytes, int32_t *status, void *stream)
+{
+    return predict(c, false, preds, n, out, out_bytes, status, stream);
+}
+
+TCME_API int tcme_predict_chroma(tcme_ctx *c, const tcme_pred *preds, int n, uint8_t *out,
+                                 int64_t out_bytes, int32_t *status, void *stream)
+{
+    return predict(c, true, preds, n, out, out_bytes, status, stream);
 }
 
 TCME_API int tcme_set_timing(tcme_ctx *c, int on)

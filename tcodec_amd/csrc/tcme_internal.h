@@ -38,6 +38,10 @@ struct Tiling {
 };
 
 bool valid_waves(int blk, int waves);
+// Motion compensation of n blocks (tcme_mc.hip): luma from the anchor plane,
+// or chroma from a plain plane.
+hipError_t launch_predict(bool chroma, const PicDev *table, int npics, const tcme_pred *preds,
+                          int n, uint8_t *out, int64_t out_bytes, int32_t *status, hipStream_t s);
 // The tile kernel: tile workgroups of `waves` wavefronts, `chunk` calls each;
 // searches the calls of size 8 and 16 in the list, or, with `big` (4
 // wavefronts), every call.

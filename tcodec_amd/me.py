@@ -28,12 +28,14 @@ EXPORTS = (
     "tcme_pic_create", "tcme_pic_destroy", "tcme_pic_upload", "tcme_pic_upload_device",
     "tcme_pic_luma", "tcme_pic_finish", "tcme_search", "tcme_search_host",
     "tcme_set_tiling", "tcme_get_tiling", "tcme_autotune", "tcme_set_timing",
-    "tcme_last_timing",
+    "tcme_last_timing", "tcme_pic_create_plane", "tcme_predict_luma", "tcme_predict_chroma",
 )
 
 CALL_DTYPE = np.dtype([("ref_pic", "<i4"), ("cur_pic", "<i4"), ("px", "<i4"), ("py", "<i4"),
                        ("cx", "<i4"), ("cy", "<i4"), ("blk", "<i4"), ("sr", "<i4")])
 RESULT_DTYPE = np.dtype([("mv_x", "<i4"), ("mv_y", "<i4"), ("sad", "<i4"), ("status", "<i4")])
+PRED_DTYPE = np.dtype([("pic", "<i4"), ("mv_x", "<i4"), ("mv_y", "<i4"), ("blk", "<i4"),
+                       ("out_off", "<i8"), ("out_stride", "<i4"), ("reserved", "<i4")])
 STATS_DTYPE = np.dtype([("int_evals", "<i4"), ("q6_evals", "<i4"), ("qbil_evals", "<i4"),
                         ("int_oob", "<i4")])
 
@@ -116,6 +118,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tcme_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                 C.c_void_p, C.POINTER(Tiling), C.POINTER(C.c_float)]
     L.tcme_set_timing.argtypes = [C.c_void_p, C.c_int]
+    L.tcme_pic_create_plane.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    for f in (L.tcme_predict_luma, L.tcme_predict_chroma):
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                      C.c_void_p]
     L.tcme_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int]
     _lib = L
     return L
@@ -181,6 +187,38 @@ class Engine:
         pid = _check(self.L.tcme_pic_create(self.h, w, h))
         self.pics[pid] = (w, h)
         return pid
+
+    def plane(self, plane: np.ndarray, stream: int = 0) -> int:
+        """New plain picture (no anchor plane) holding a 2-D uint8 plane."""
+        plane = np.ascontiguousarray(plane, np.uint8)
+        hh, ww = plane.shape
+        pid = _check(self.L.tcme_pic_create_plane(self.h, ww, hh))
+        self.pics[pid] = (ww, hh)
+        self.upload(pid, plane, stream)
+        return pid
+
+    def predict_device(self, preds_ptr: int, n: int, out_ptr: int, out_bytes: int,
+                       status_ptr: int | None = None, chroma: bool = False, stream: int = 0):
+        """Device descriptors (PRED_DTYPE) -> predicted blocks in a device
+        buffer; asynchronous on ``stream``."""
+        f = self.L.tcme_predict_chroma if chroma else self.L.tcme_predict_luma
+        _check(f(self.h, C.c_void_p(preds_ptr), n, C.c_void_p(out_ptr), int(out_bytes),
+                 C.c_void_p(status_ptr) if status_ptr else None, C.c_void_p(stream)))
+
+    def predict(self, preds: np.ndarray, chroma: bool = False):
+        """Host descriptors -> (uint8 output buffer, status per block)."""
+        import torch
+        preds = np.ascontiguousarray(preds, PRED_DTYPE)
+        end = preds["out_off"] + (preds["blk"] - 1).astype(np.int64) * preds["out_stride"] + \
+            preds["blk"]
+        nbytes = int(max(end.max(), 1))
+        d_p = torch.from_numpy(preds.view(np.uint8).copy()).cuda()
+        d_o = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
+        d_s = torch.zeros(len(preds), dtype=torch.int32, device="cuda")
+        self.predict_device(d_p.data_ptr(), len(preds), d_o.data_ptr(), nbytes, d_s.data_ptr(),
+                            chroma)
+        torch.cuda.synchronize()
+        return d_o.cpu().numpy(), d_s.cpu().numpy()
 
     def upload(self, pid: int, plane: np.ndarray, stream: int = 0) -> None:
         plane = np.ascontiguousarray(plane, np.uint8)

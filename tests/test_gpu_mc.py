@@ -1,0 +1,84 @@
+"""GPU motion compensation (tcme_predict_luma / tcme_predict_chroma) against
+the oracle's tc_inter_predict / tc_inter_predict_chroma, block for block."""
+import numpy as np
+import pytest
+
+from helpers import moving_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def _preds(pic, mvs, sizes, stride=None):
+    from tcodec_amd import me
+    p = np.zeros(len(mvs), me.PRED_DTYPE)
+    p["pic"] = pic
+    p["mv_x"], p["mv_y"] = mvs[:, 0], mvs[:, 1]
+    p["blk"] = sizes
+    p["out_stride"] = sizes if stride is None else stride
+    p["out_off"] = np.concatenate([[0], np.cumsum(sizes.astype(np.int64) ** 2)[:-1]]) \
+        if stride is None else np.arange(len(mvs)) * stride * 64
+    return p
+
+
+@pytest.mark.parametrize("shape", [(61, 97), (128, 160), (40, 64), (72, 200)])
+def test_gpu_luma_mc_matches_oracle(engine, orc, shape):
+    h, w = shape
+    print(f"luma mc {w}x{h}", flush=True)
+    rng = np.random.default_rng(h * 31 + w)
+    ref, _ = moving_pair(h, w, 5)
+    rp = engine.pic(ref)
+    sizes = rng.choice([1, 2, 4, 8, 16, 32, 64], 700)
+    # positions over the frame and up to 40 samples beyond every edge, any phase
+    mvs = np.stack([rng.integers(-160, 4 * w + 160, 700), rng.integers(-160, 4 * h + 160, 700)], 1)
+    mvs[:60] &= ~3                                   # integer MVs too
+    preds = _preds(rp, mvs, sizes)
+    out, st = engine.predict(preds)
+    assert (st == 0).all()
+    for k in range(len(preds)):
+        n = int(sizes[k])
+        o = int(preds["out_off"][k])
+        got = out[o:o + n * n].reshape(n, n)
+        want = orc.inter_predict(ref, int(mvs[k, 0]), int(mvs[k, 1]), n)
+        assert np.array_equal(got, want), (k, mvs[k], n)
+    engine.free(rp)
+
+
+@pytest.mark.parametrize("shape", [(30, 48), (64, 80), (20, 33)])
+def test_gpu_chroma_mc_matches_oracle(engine, orc, shape):
+    h, w = shape
+    print(f"chroma mc {w}x{h}", flush=True)
+    rng = np.random.default_rng(h * 7 + w)
+    pl, _ = moving_pair(h, w, 9)
+    cp = engine.plane(pl)
+    sizes = rng.choice([1, 2, 4, 8, 16, 32], 600)
+    mvs = np.stack([rng.integers(-200, 8 * w + 200, 600), rng.integers(-200, 8 * h + 200, 600)], 1)
+    mvs[:50] &= ~7
+    preds = _preds(cp, mvs, sizes)
+    out, st = engine.predict(preds, chroma=True)
+    assert (st == 0).all()
+    for k in range(len(preds)):
+        n = int(sizes[k])
+        o = int(preds["out_off"][k])
+        got = out[o:o + n * n].reshape(n, n)
+        want = orc.inter_predict_chroma(pl, int(mvs[k, 0]), int(mvs[k, 1]), n)
+        assert np.array_equal(got, want), (k, mvs[k], n)
+    engine.free(cp)
+
+
+def test_gpu_mc_strided_output_and_bad_descriptors(engine, orc):
+    from tcodec_amd import me
+    ref, _ = moving_pair(64, 96, 2)
+    rp, pp = engine.pic(ref), engine.plane(ref)
+    mvs = np.array([[37, 41], [100, 9], [8, 8], [5, 5], [5, 5], [5, 5]])
+    p = _preds(rp, mvs, np.array([8, 16, 8, 8, 0, 65]), stride=80)
+    p["pic"][2] = 999      # unknown picture
+    p["pic"][3] = pp       # a plain plane cannot be a luma reference
+    out, st = engine.predict(p)
+    assert st.tolist() == [0, 0, -4, -4, -5, -5]
+    for k in (0, 1):
+        n = int(p["blk"][k])
+        o = int(p["out_off"][k])
+        got = np.stack([out[o + r * 80:o + r * 80 + n] for r in range(n)])
+        assert np.array_equal(got, orc.inter_predict(ref, int(mvs[k, 0]), int(mvs[k, 1]), n))
+    engine.free(rp)
+    engine.free(pp)
