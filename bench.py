@@ -266,6 +266,55 @@ def mc_leg(eng, frames, d_res, reps: int = 5):
     return out
 
 
+def aux_leg(eng, frames, pics, reps: int = 3):
+    """Global-motion estimates for every timed picture against its nearest
+    reference, and the co-located 8x8 and 64x64 stability SADs of one picture
+    against the previous one, each as one launch."""
+    import torch
+    from tcodec_amd import me
+    pairs = np.array([(int(c["cur_pic"][0]), int(np.asarray(c["ref_pic"])[0]))
+                      for _, _, c in frames], np.int32)
+    d_pairs = torch.from_numpy(pairs).cuda()
+    d_out = torch.zeros((len(pairs), 3), dtype=torch.int32, device="cuda")
+    s = torch.cuda.current_stream()
+
+    def gmv():
+        me._check(eng.L.tcme_global_mv(eng.h, me.C.c_void_p(d_pairs.data_ptr()), len(pairs),
+                                       me.C.c_void_p(d_out.data_ptr()), me.C.c_void_p(s.cuda_stream)))
+
+    descs = []
+    cur, prev = pics[8], pics[7]
+    for blk in (8, 64):
+        for y in range(0, H, blk):
+            for x in range(0, W, blk):
+                descs.append((cur, x, y, min(blk, W - x), min(blk, H - y), prev, x, y, 0, 0, 0))
+    d = np.array(descs, me.SAD_DTYPE)
+    d_d = torch.from_numpy(d.view(np.uint8).copy()).cuda()
+    d_s = torch.zeros(len(d), dtype=torch.int32, device="cuda")
+
+    def sads():
+        me._check(eng.L.tcme_sad(eng.h, me.C.c_void_p(d_d.data_ptr()), len(d), None, 0,
+                                 me.C.c_void_p(d_s.data_ptr()), me.C.c_void_p(s.cuda_stream)))
+
+    out = {}
+    for name, fn in (("global_mv", gmv), ("stability_sad", sads)):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        out[name] = round(e0.elapsed_time(e1) / reps, 3)
+    assert (d_out[:, 2] == 0).all().item() and (d_s >= 0).all().item()
+    res = {"global_mv": {"pairs": len(pairs), "ms": out["global_mv"],
+                         "ms_per_pair": round(out["global_mv"] / len(pairs), 3)},
+           "stability_sad": {"blocks": len(d), "ms": out["stability_sad"],
+                             "GBs": round(2 * W * H * 2 / out["stability_sad"] / 1e6, 1)}}
+    log(f"aux: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,6 +454,7 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(scene, pics, frames)
     mc = mc_leg(eng, frames, d_res)
+    aux = aux_leg(eng, frames, pics)
     dropin = None if args.no_forms else dropin_rate(frames, scene, pics)
     batched = None if args.no_forms else batched_form(eng, args.batch_steps)
     served = None if args.no_forms else served_form(eng, args.serve_rate, args.serve_seconds,
@@ -449,6 +499,7 @@ def main():
                              "note": "tc_motion_est, one launch per call, host planes"}
                             if dropin else None),
         "mc": mc,
+        "aux": aux,
         "batched": batched,
         "served": served,
     }

@@ -157,6 +157,33 @@ int  tcme_predict_luma(tcme_ctx *ctx, const tcme_pred *preds, int n, uint8_t *ou
 int  tcme_predict_chroma(tcme_ctx *ctx, const tcme_pred *preds, int n, uint8_t *out,
                          int64_t out_bytes, int32_t *status, void *stream);
 
+/* ---- encoder-side helpers (batched) ------------------------------------ */
+
+/* Replaces estimate_global_mv (reference src/encoder.c:171-202) for n
+ * (current, reference) picture pairs of one size: pairs holds 2 ids per pair,
+ * out receives 3 ints per pair (glob_mv_x, glob_mv_y in quarter-pel, status).
+ * Device arrays, asynchronous on stream. */
+int  tcme_global_mv(tcme_ctx *ctx, const int32_t *pairs, int n, int32_t *out, void *stream);
+
+/* A block SAD: the w x h block (1..64 each) of picture pic at (x,y) against
+ * the block of picture other_pic at (ox,oy), or, with other_pic < 0, against
+ * the caller's buffer at buf_off with row stride buf_stride.  Covers the SAD
+ * proxies of the reference encoder (tc_sad of a prediction,
+ * src/encoder.c:814, 839, 955, 1171) and its co-located stability SADs
+ * (src/encoder.c:2843-2891). */
+typedef struct tcme_sad_desc {
+    int32_t pic, x, y, w, h;
+    int32_t other_pic, ox, oy;
+    int64_t buf_off;
+    int32_t buf_stride, reserved;
+} tcme_sad_desc;
+
+/* n SADs into out (>= 0, or a negative TCME_* code for a bad descriptor);
+ * descs, buf (may be NULL when no descriptor uses it, buf_bytes long) and
+ * out are device arrays.  Asynchronous on stream. */
+int  tcme_sad(tcme_ctx *ctx, const tcme_sad_desc *descs, int n, const uint8_t *buf,
+              int64_t buf_bytes, int32_t *out, void *stream);
+
 /* Per-kernel timing of dispatches (events on the stream each kernel runs
  * on).  tcme_last_timing waits for the last dispatch and writes the duration
  * in ms of each kernel it launched, in launch order: the 64x64 and 32x32

@@ -37,6 +37,12 @@ class Stats(C.Structure):
 
 
 def ptr(a: np.ndarray, typ=u8p):
+    """Data pointer of `a`.  The wrappers pass ``a.strides[0]`` as the row
+    pitch, so a 2-D plane must be row-major with contiguous rows (a
+    column-major array, e.g. what ``np.roll`` over both axes can return,
+    would be read transposed)."""
+    if a.ndim >= 2 and (a.strides[-1] != a.itemsize or a.strides[0] < a.shape[-1] * a.itemsize):
+        raise ValueError(f"plane must be row-major with contiguous rows, strides {a.strides}")
     return a.ctypes.data_as(typ)
 
 

@@ -471,6 +471,29 @@ TCME_API int tcme_predict_chroma(tcme_ctx *c, const tcme_pred *preds, int n, uin
     return predict(c, true, preds, n, out, out_bytes, status, stream);
 }
 
+TCME_API int tcme_global_mv(tcme_ctx *c, const int32_t *pairs, int n, int32_t *out, void *stream)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    if (n < 0 || (n > 0 && (!pairs || !out))) return fail(TCME_E_ARG, "bad pair list");
+    if (n == 0) return TCME_OK;
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    HIP_TRY(tcme::launch_global_mv(c->d_table, c->max_pics, pairs, n, out,
+                                   static_cast<hipStream_t>(stream)));
+    return TCME_OK;
+}
+
+TCME_API int tcme_sad(tcme_ctx *c, const tcme_sad_desc *descs, int n, const uint8_t *buf,
+                      int64_t buf_bytes, int32_t *out, void *stream)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    if (n < 0 || (n > 0 && (!descs || !out))) return fail(TCME_E_ARG, "bad SAD list");
+    if (n == 0) return TCME_OK;
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    HIP_TRY(tcme::launch_sad(c->d_table, c->max_pics, descs, n, buf, buf ? buf_bytes : 0, out,
+                             static_cast<hipStream_t>(stream)));
+    return TCME_OK;
+}
+
 TCME_API int tcme_set_timing(tcme_ctx *c, int on)
 {
     if (!c) return fail(TCME_E_ARG, "null context");

@@ -38,6 +38,11 @@ struct Tiling {
 };
 
 bool valid_waves(int blk, int waves);
+// Global-motion estimates and block SADs (tcme_aux.hip).
+hipError_t launch_global_mv(const PicDev *table, int npics, const int32_t *pairs, int n,
+                            int32_t *out, hipStream_t s);
+hipError_t launch_sad(const PicDev *table, int npics, const tcme_sad_desc *descs, int n,
+                      const uint8_t *buf, int64_t buf_bytes, int32_t *out, hipStream_t s);
 // Motion compensation of n blocks (tcme_mc.hip): luma from the anchor plane,
 // or chroma from a plain plane.
 hipError_t launch_predict(bool chroma, const PicDev *table, int npics, const tcme_pred *preds,

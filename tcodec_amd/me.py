@@ -29,6 +29,7 @@ EXPORTS = (
     "tcme_pic_luma", "tcme_pic_finish", "tcme_search", "tcme_search_host",
     "tcme_set_tiling", "tcme_get_tiling", "tcme_autotune", "tcme_set_timing",
     "tcme_last_timing", "tcme_pic_create_plane", "tcme_predict_luma", "tcme_predict_chroma",
+    "tcme_global_mv", "tcme_sad",
 )
 
 CALL_DTYPE = np.dtype([("ref_pic", "<i4"), ("cur_pic", "<i4"), ("px", "<i4"), ("py", "<i4"),
@@ -36,6 +37,9 @@ CALL_DTYPE = np.dtype([("ref_pic", "<i4"), ("cur_pic", "<i4"), ("px", "<i4"), ("
 RESULT_DTYPE = np.dtype([("mv_x", "<i4"), ("mv_y", "<i4"), ("sad", "<i4"), ("status", "<i4")])
 PRED_DTYPE = np.dtype([("pic", "<i4"), ("mv_x", "<i4"), ("mv_y", "<i4"), ("blk", "<i4"),
                        ("out_off", "<i8"), ("out_stride", "<i4"), ("reserved", "<i4")])
+SAD_DTYPE = np.dtype([("pic", "<i4"), ("x", "<i4"), ("y", "<i4"), ("w", "<i4"), ("h", "<i4"),
+                      ("other_pic", "<i4"), ("ox", "<i4"), ("oy", "<i4"), ("buf_off", "<i8"),
+                      ("buf_stride", "<i4"), ("reserved", "<i4")])
 STATS_DTYPE = np.dtype([("int_evals", "<i4"), ("q6_evals", "<i4"), ("qbil_evals", "<i4"),
                         ("int_oob", "<i4")])
 
@@ -119,6 +123,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
                                 C.c_void_p, C.POINTER(Tiling), C.POINTER(C.c_float)]
     L.tcme_set_timing.argtypes = [C.c_void_p, C.c_int]
     L.tcme_pic_create_plane.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.tcme_global_mv.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    L.tcme_sad.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                           C.c_void_p]
     for f in (L.tcme_predict_luma, L.tcme_predict_chroma):
         f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
                       C.c_void_p]
@@ -204,6 +211,30 @@ class Engine:
         f = self.L.tcme_predict_chroma if chroma else self.L.tcme_predict_luma
         _check(f(self.h, C.c_void_p(preds_ptr), n, C.c_void_p(out_ptr), int(out_bytes),
                  C.c_void_p(status_ptr) if status_ptr else None, C.c_void_p(stream)))
+
+    def global_mv(self, pairs) -> np.ndarray:
+        """[(cur_pic, ref_pic)] -> int32 [n, 3] = (glob_mv_x, glob_mv_y, status)."""
+        import torch
+        p = torch.tensor(np.asarray(pairs, np.int32).reshape(-1, 2)).cuda()
+        out = torch.zeros((p.shape[0], 3), dtype=torch.int32, device="cuda")
+        _check(self.L.tcme_global_mv(self.h, C.c_void_p(p.data_ptr()), p.shape[0],
+                                     C.c_void_p(out.data_ptr()), None))
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+    def sad(self, descs: np.ndarray, buf: np.ndarray | None = None) -> np.ndarray:
+        """Host SAD descriptors (SAD_DTYPE) [+ host buffer] -> int32 SADs."""
+        import torch
+        descs = np.ascontiguousarray(descs, SAD_DTYPE)
+        d = torch.from_numpy(descs.view(np.uint8).copy()).cuda()
+        b = torch.from_numpy(np.ascontiguousarray(buf, np.uint8)).cuda() if buf is not None else None
+        out = torch.zeros(len(descs), dtype=torch.int32, device="cuda")
+        _check(self.L.tcme_sad(self.h, C.c_void_p(d.data_ptr()), len(descs),
+                               C.c_void_p(b.data_ptr()) if b is not None else None,
+                               int(b.numel()) if b is not None else 0,
+                               C.c_void_p(out.data_ptr()), None))
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
 
     def predict(self, preds: np.ndarray, chroma: bool = False):
         """Host descriptors -> (uint8 output buffer, status per block)."""

@@ -33,7 +33,9 @@ def smooth_plane(h, w, seed, cell=8):
     fx = (xx - x0)[None, :]
     a, b = g[y0][:, x0], g[y0][:, x0 + 1]
     c, d = g[y0 + 1][:, x0], g[y0 + 1][:, x0 + 1]
-    return a * (1 - fx) * (1 - fy) + b * fx * (1 - fy) + c * (1 - fx) * fy + d * fx * fy
+    # the fancy indexing above yields column-major arrays: hand back row-major
+    return np.ascontiguousarray(a * (1 - fx) * (1 - fy) + b * fx * (1 - fy) +
+                                c * (1 - fx) * fy + d * fx * fy)
 
 
 def moving_pair(h, w, seed, shift=(3, -5), noise=3):

@@ -15,13 +15,14 @@ from conftest import ROOT
 CSRC = os.path.join(ROOT, "tcodec_amd", "csrc")
 ASM = os.path.join(ROOT, "tcodec_amd", "lib", "tcme_kernels.s")
 ASM_MC = os.path.join(ROOT, "tcodec_amd", "lib", "tcme_mc.s")
+ASM_AUX = os.path.join(ROOT, "tcodec_amd", "lib", "tcme_aux.s")
 SIZE = {"b32": 4, "b64": 8, "b96": 16, "b128": 16, "u16": 2, "i16": 2, "b16": 2, "u8": 1,
         "i8": 1, "b8": 1}
 
 
 def assembly() -> str:
     subprocess.run(["make", "-s", "-C", CSRC, "asm"], check=True)
-    return open(ASM).read() + open(ASM_MC).read()
+    return open(ASM).read() + open(ASM_MC).read() + open(ASM_AUX).read()
 
 
 def test_lds_accesses_are_naturally_aligned():
@@ -57,3 +58,4 @@ def test_every_kernel_present():
         assert re.search(rf"me_tile_kernelILi{w}E", text), w
     assert "anchor_kernel" in text
     assert "mc_luma_kernel" in text and "mc_chroma_kernel" in text
+    assert "global_mv_kernel" in text and "sad_kernel" in text
