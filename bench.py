@@ -334,6 +334,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="arrival window of the served run (the serving config uses 120)")
     ap.add_argument("--serve-budget-gib", type=float, default=48.0)
+    ap.add_argument("--launch", choices=("graph", "direct"), default="graph",
+                    help="graph: per-frame searches captured as HIP graphs and replayed "
+                         "(the value); direct: one tcme_search call per frame")
     ap.add_argument("--mode", choices=("tuned", "default"), default="tuned",
                     help="default: the built-in launch shape only; tuned: also the autotuned "
                          "shape (reported as value)")
@@ -370,12 +373,17 @@ def main():
         d_res.append(torch.zeros((len(calls), 4), dtype=torch.int32, device=dev))
         d_stats.append(torch.zeros((len(calls), 4), dtype=torch.int32, device=dev))
 
+    graphs = []  # per-frame captured searches (tcme_graph), when replaying
+
     def step(with_stats=False, events=None):
         for i, (poc, kind, calls) in enumerate(frames):
             if events is not None:
                 events[i][0].record(stream)
-            eng.search_device(d_calls[i].data_ptr(), len(calls), d_res[i].data_ptr(),
-                              d_stats[i].data_ptr() if with_stats else None, sptr)
+            if graphs:
+                graphs[i].launch(sptr)
+            else:
+                eng.search_device(d_calls[i].data_ptr(), len(calls), d_res[i].data_ptr(),
+                                  d_stats[i].data_ptr() if with_stats else None, sptr)
             if events is not None:
                 events[i][1].record(stream)
 
@@ -435,6 +443,19 @@ def main():
         best = eng.tiling()
         elapsed, lat = plain_el, plain_lat
     nframes = len(frames) * args.steps
+    direct = {"frames_per_s": round(nframes / elapsed, 2),
+              "p50_ms": round(float(np.percentile(lat, 50)), 3),
+              "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+    if args.launch == "graph":
+        # each frame's search captured once (chosen shape), replayed per step
+        for i, (poc, kind, calls) in enumerate(frames):
+            graphs.append(eng.graph(d_calls[i].data_ptr(), len(calls), d_res[i].data_ptr()))
+        elapsed, lat = timed("graph replay")
+        for g in graphs:
+            g.close()
+        graphs.clear()
+        for i in range(len(frames)):
+            assert (d_res[i][:, 3] == 0).all().item()
     fps = nframes / elapsed
     calls_per_frame = int(np.mean([len(c) for _, _, c in frames]))
     # the dominant (longest) kernel: its own calls' algorithmic bytes over its
@@ -479,6 +500,8 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),
                        "load": "back-to-back per-frame launches, one stream"},
+        "launch": args.launch,
+        "direct_launch": direct,
         "default_shape": {"value": round(nframes / plain_el, 2),
                           "p50_ms": round(float(np.percentile(plain_lat, 50)), 3)},
         "tiling": best.as_dict(),

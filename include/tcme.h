@@ -82,14 +82,15 @@ const char *tcme_version(void);
 int  tcme_create(tcme_ctx **out, int device, int max_pics);
 void tcme_destroy(tcme_ctx *ctx);
 
-/* Allocates a w x h luma picture with its half-sample planes on the device. */
+/* Allocates a w x h luma picture with its anchor plane (per sample: the sample
+ * and its raw six-tap half samples right, below and diagonal) on the device. */
 int  tcme_pic_create(tcme_ctx *ctx, int w, int h);
 /* A plain w x h plane with no anchor plane: current pictures and chroma
  * planes (it cannot be a search or luma-prediction reference). */
 int  tcme_pic_create_plane(tcme_ctx *ctx, int w, int h);
 int  tcme_pic_destroy(tcme_ctx *ctx, int pic);
-/* Fills a picture from host (upload) or device memory, then derives its raw
- * 6-tap half-sample planes.  Asynchronous on stream (a hipStream_t). */
+/* Fills a picture from host (upload) or device memory, then rebuilds its
+ * anchor plane (if it has one).  Asynchronous on stream (a hipStream_t). */
 int  tcme_pic_upload(tcme_ctx *ctx, int pic, const uint8_t *host, int stride, void *stream);
 int  tcme_pic_upload_device(tcme_ctx *ctx, int pic, const uint8_t *dev, int stride,
                             void *stream);
@@ -132,6 +133,19 @@ int  tcme_get_tiling(tcme_ctx *ctx, tcme_tiling *t);
 int  tcme_autotune(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,
                    int reps, void *stream, tcme_tiling *best, float *best_ms);
 
+/* A search captured once as a HIP graph and replayed per frame: the launches
+ * tcme_search makes for (calls, n, results, stats) with the context's launch
+ * shape at capture time, replayed with no per-launch host work.  The buffers
+ * are read when a replay runs, so the caller rewrites the call list (same n)
+ * and the pictures between replays.  Replays on one graph must not overlap,
+ * and per-kernel timing (tcme_set_timing) does not see them.  The graph
+ * belongs to the context: destroy it first. */
+typedef struct tcme_graph tcme_graph;
+int  tcme_graph_create(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,
+                       tcme_stats *stats, tcme_graph **out);
+int  tcme_graph_launch(tcme_graph *g, void *stream);
+void tcme_graph_destroy(tcme_graph *g);
+
 /* ---- motion compensation (batched) -------------------------------------- */
 
 /* One predicted block: picture id, absolute motion vector (quarter sample
@@ -162,7 +176,10 @@ int  tcme_predict_chroma(tcme_ctx *ctx, const tcme_pred *preds, int n, uint8_t *
 /* Replaces estimate_global_mv (reference src/encoder.c:171-202) for n
  * (current, reference) picture pairs of one size: pairs holds 2 ids per pair,
  * out receives 3 ints per pair (glob_mv_x, glob_mv_y in quarter-pel, status).
- * Device arrays, asynchronous on stream. */
+ * Device arrays, asynchronous on stream; n <= 65535.  The context keeps a
+ * workspace for the coarse SADs of every displacement within +-128 px (grown
+ * on first use, freed with the context); successive calls on one context are
+ * ordered on the device whatever their streams. */
 int  tcme_global_mv(tcme_ctx *ctx, const int32_t *pairs, int n, int32_t *out, void *stream);
 
 /* A block SAD: the w x h block (1..64 each) of picture pic at (x,y) against

@@ -69,6 +69,12 @@ struct tcme_ctx {
     tcme_result *d_res = nullptr;
     tcme_stats *d_stats = nullptr;
     size_t stage_cap = 0;
+    // global-motion partial table (grown on demand); successive estimates on
+    // one context are ordered by gmv_done, since they share it
+    int32_t *d_gmv = nullptr;
+    size_t gmv_cap = 0;
+    hipEvent_t gmv_done = nullptr;
+    bool gmv_pending = false;
     tcme::Tiling tiling;
     // plain form: the four per-size kernels run concurrently on these streams,
     // forked from and joined back into the caller's stream with events
@@ -129,14 +135,21 @@ int pic_alloc(tcme_ctx *c, int w, int h, bool half)
     size_t ybytes = static_cast<size_t>(p.pitch) * h + tcme::kTailSlack;
     if (hipMalloc(&p.d_y, ybytes) != hipSuccess)
         return fail(TCME_E_NOMEM, "hipMalloc luma %zu bytes", ybytes);
-    hipMemset(p.d_y, 0, ybytes);
+    if (hipMemset(p.d_y, 0, ybytes) != hipSuccess) {
+        (void)hipFree(p.d_y);
+        return fail(TCME_E_HIP, "clearing luma");
+    }
     if (half) {
         size_t abytes = static_cast<size_t>(p.apitch) * h * sizeof(uint64_t) + tcme::kTailSlack;
         if (hipMalloc(&p.d_an, abytes) != hipSuccess) {
-            hipFree(p.d_y);
+            (void)hipFree(p.d_y);
             return fail(TCME_E_NOMEM, "hipMalloc anchor plane %zu bytes", abytes);
         }
-        hipMemset(p.d_an, 0, abytes);
+        if (hipMemset(p.d_an, 0, abytes) != hipSuccess) {
+            (void)hipFree(p.d_y);
+            (void)hipFree(p.d_an);
+            return fail(TCME_E_HIP, "clearing anchor plane");
+        }
     }
     p.used = true;
     c->pics[id] = p;
@@ -148,8 +161,8 @@ int pic_alloc(tcme_ctx *c, int w, int h, bool half)
 int pic_free(tcme_ctx *c, int id)
 {
     Pic &p = c->pics[id];
-    hipFree(p.d_y);
-    if (p.d_an) hipFree(p.d_an);
+    (void)hipFree(p.d_y);
+    if (p.d_an) (void)hipFree(p.d_an);
     p = Pic();
     tcme::PicDev d{};
     HIP_TRY(hipMemcpy(c->d_table + id, &d, sizeof d, hipMemcpyHostToDevice));
@@ -228,8 +241,9 @@ TCME_API int tcme_create(tcme_ctx **out, int device, int max_pics)
         delete c;
         return fail(TCME_E_NOMEM, "picture table");
     }
-    hipMemset(c->d_table, 0, sizeof(tcme::PicDev) * max_pics);
-    bool ok = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess;
+    bool ok = hipMemset(c->d_table, 0, sizeof(tcme::PicDev) * max_pics) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->gmv_done, hipEventDisableTiming) == hipSuccess;
     for (int k = 0; k < 4 && ok; ++k)
         ok = hipStreamCreateWithFlags(&c->sub[k], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming) == hipSuccess &&
@@ -245,21 +259,23 @@ TCME_API int tcme_create(tcme_ctx **out, int device, int max_pics)
 TCME_API void tcme_destroy(tcme_ctx *c)
 {
     if (!c) return;
-    hipSetDevice(c->device);
-    hipDeviceSynchronize();
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
     for (int i = 0; i < c->max_pics; ++i)
         if (c->pics[i].used) pic_free(c, i);
-    hipFree(c->d_table);
-    hipFree(c->d_calls);
-    hipFree(c->d_res);
-    hipFree(c->d_stats);
+    (void)hipFree(c->d_table);
+    (void)hipFree(c->d_calls);
+    (void)hipFree(c->d_res);
+    (void)hipFree(c->d_stats);
     for (int k = 0; k < 4; ++k) {
-        if (c->sub[k]) hipStreamDestroy(c->sub[k]);
-        if (c->join[k]) hipEventDestroy(c->join[k]);
-        if (c->t0[k]) hipEventDestroy(c->t0[k]);
-        if (c->t1[k]) hipEventDestroy(c->t1[k]);
+        if (c->sub[k]) (void)hipStreamDestroy(c->sub[k]);
+        if (c->join[k]) (void)hipEventDestroy(c->join[k]);
+        if (c->t0[k]) (void)hipEventDestroy(c->t0[k]);
+        if (c->t1[k]) (void)hipEventDestroy(c->t1[k]);
     }
-    if (c->fork) hipEventDestroy(c->fork);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->gmv_done) (void)hipEventDestroy(c->gmv_done);
+    (void)hipFree(c->d_gmv);
     delete c;
 }
 
@@ -442,6 +458,67 @@ TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_resu
     return TCME_OK;
 }
 
+struct tcme_graph {
+    tcme_ctx *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+// Captures dispatch() on a private stream: the fork/join events pull the
+// context's side streams into the capture, so the graph holds the same
+// concurrent kernels.  dispatch() makes no allocation, copy or memset.
+TCME_API int tcme_graph_create(tcme_ctx *c, const tcme_call *calls, int n, tcme_result *results,
+                               tcme_stats *stats, tcme_graph **out)
+{
+    if (!c || !out) return fail(TCME_E_ARG, "null argument");
+    *out = nullptr;
+    if (n <= 0 || !calls || !results) return fail(TCME_E_ARG, "bad call list");
+    std::lock_guard<std::mutex> g(c->mu);
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    hipStream_t cs = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    const bool timing = c->timing;
+    c->timing = false;
+    hipGraph_t graph = nullptr;
+    int err = TCME_OK;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        err = fail(TCME_E_HIP, "begin capture");
+    } else {
+        err = dispatch(c, calls, n, c->tiling, results, stats, cs);
+        const hipError_t e = hipStreamEndCapture(cs, &graph);
+        if (!err && e != hipSuccess) err = fail(TCME_E_HIP, "end capture: %s", hipGetErrorString(e));
+    }
+    c->timing = timing;
+    (void)hipStreamDestroy(cs);
+    hipGraphExec_t exec = nullptr;
+    if (!err && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+        err = fail(TCME_E_HIP, "graph instantiate");
+    if (err) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return err;
+    }
+    *out = new tcme_graph{c, graph, exec};
+    return TCME_OK;
+}
+
+TCME_API int tcme_graph_launch(tcme_graph *g, void *stream)
+{
+    if (!g) return fail(TCME_E_ARG, "null graph");
+    if (set_device(g->ctx) != TCME_OK) return TCME_E_HIP;
+    HIP_TRY(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+    return TCME_OK;
+}
+
+TCME_API void tcme_graph_destroy(tcme_graph *g)
+{
+    if (!g) return;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipDeviceSynchronize();
+    (void)hipGraphExecDestroy(g->exec);
+    (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
 namespace {
 
 int predict(tcme_ctx *c, bool chroma, const tcme_pred *preds, int n, uint8_t *out,
@@ -475,10 +552,30 @@ TCME_API int tcme_global_mv(tcme_ctx *c, const int32_t *pairs, int n, int32_t *o
 {
     if (!c) return fail(TCME_E_ARG, "null context");
     if (n < 0 || (n > 0 && (!pairs || !out))) return fail(TCME_E_ARG, "bad pair list");
+    if (n > 65535) return fail(TCME_E_ARG, "%d pairs in one call (at most 65535)", n);
     if (n == 0) return TCME_OK;
+    std::lock_guard<std::mutex> g(c->mu);
     if (set_device(c) != TCME_OK) return TCME_E_HIP;
-    HIP_TRY(tcme::launch_global_mv(c->d_table, c->max_pics, pairs, n, out,
-                                   static_cast<hipStream_t>(stream)));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int max_w = 1, max_h = 1;
+    for (const Pic &p : c->pics)
+        if (p.used) { max_w = std::max(max_w, p.w); max_h = std::max(max_h, p.h); }
+    const size_t need = tcme::gmv_part_bytes(n, max_w, max_h);
+    if (need > c->gmv_cap) {
+        if (c->gmv_pending) HIP_TRY(hipEventSynchronize(c->gmv_done));
+        HIP_TRY(hipFree(c->d_gmv));
+        c->d_gmv = nullptr;
+        c->gmv_cap = 0;
+        if (hipMalloc(&c->d_gmv, need) != hipSuccess)
+            return fail(TCME_E_NOMEM, "global-motion table %zu bytes", need);
+        c->gmv_cap = need;
+    } else if (c->gmv_pending) {
+        HIP_TRY(hipStreamWaitEvent(s, c->gmv_done, 0));
+    }
+    HIP_TRY(tcme::launch_global_mv(c->d_table, c->max_pics, pairs, n, max_w, max_h, c->d_gmv, out,
+                                   s));
+    HIP_TRY(hipEventRecord(c->gmv_done, s));
+    c->gmv_pending = true;
     return TCME_OK;
 }
 
@@ -526,9 +623,9 @@ TCME_API int tcme_search_host(tcme_ctx *c, const tcme_call *calls, int n, tcme_r
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (static_cast<size_t>(n) > c->stage_cap) {
         HIP_TRY(hipStreamSynchronize(s));
-        hipFree(c->d_calls);
-        hipFree(c->d_res);
-        hipFree(c->d_stats);
+        (void)hipFree(c->d_calls);
+        (void)hipFree(c->d_res);
+        (void)hipFree(c->d_stats);
         c->d_calls = nullptr; c->d_res = nullptr; c->d_stats = nullptr; c->stage_cap = 0;
         size_t cap = std::max<size_t>(n, 4096);
         if (hipMalloc(&c->d_calls, cap * sizeof(tcme_call)) != hipSuccess ||
@@ -626,7 +723,7 @@ int mirror_for(const uint8_t *ref, int stride, int w, int h, int x0, int y0, int
                                    [](const Mirror &a, const Mirror &b) { return a.last_use < b.last_use; });
         }
         if (m->pic >= 0 && (m->w != w || m->h != h)) {
-            hipStreamSynchronize(d.stream);
+            if (hipStreamSynchronize(d.stream) != hipSuccess) die("stream synchronize");
             pic_free(d.ctx, m->pic);
             m->pic = -1;
         }

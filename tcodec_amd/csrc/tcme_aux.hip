@@ -1,12 +1,15 @@
 // gfx950 kernels for the encoder-side helpers around the search:
 //
-//   global_mv_kernel  estimate_global_mv, reference src/encoder.c:171-202: a
-//                     greedy two-pass walk over whole-frame coarse SADs (the
+//   gmv_table_kernel  estimate_global_mv, reference src/encoder.c:171-202: a
+//   gmv_walk_kernel   greedy two-pass walk over whole-frame coarse SADs (the
 //                     ME centre hint used when |glob| > 16 quarter-pel,
-//                     src/encoder.c:1000-1004)
+//                     src/encoder.c:1000-1004): a parallel table of the SADs
+//                     of every displacement within reach, then the walk
 //   sad_kernel        block SADs: the fast-mode SAD proxies of a prediction
 //                     (src/encoder.c:814, 839, 955, 1171) and the co-located
 //                     CTU / 8x8 stability SADs (src/encoder.c:2843-2891)
+#include <algorithm>
+
 #include "tcme_internal.h"
 
 namespace tcme {
@@ -15,6 +18,10 @@ namespace {
 typedef const __attribute__((address_space(1))) uint8_t *gptr8;
 
 constexpr int kGmvThreads = 512;
+constexpr int kGmvReach = 128;                    // table covers |Dx|, |Dy| <= 128 px
+constexpr int kGmvSide = 2 * kGmvReach / 4 + 1;   // 65 displacements per axis
+constexpr int kGmvEntries = kGmvSide * kGmvSide;  // 4225
+constexpr int kGmvLdsMax = 48 * 1024;             // table kernel's staging bound
 
 __device__ __forceinline__ int64_t block_reduce(int v, int *red)
 {
@@ -36,37 +43,126 @@ __device__ __forceinline__ int64_t block_reduce(int v, int *red)
     return t;
 }
 
-// One workgroup per (current, reference) pair.  Samples sit every 8 pixels
-// (every other position of the 1/4-resolution grid, src/encoder.c:184-185);
-// a sample displaced off the frame costs 255*4 (src/encoder.c:190).  The
-// walk is the reference's: candidates in row-major order around the moving
-// best, the first of each pass always replacing the initial 2^60 bound.
-__global__ __launch_bounds__(kGmvThreads) void global_mv_kernel(const PicDev *__restrict__ table,
-                                                                int npics,
-                                                                const int32_t *__restrict__ pairs,
-                                                                int n, int32_t *__restrict__ out)
+// Coarse SAD of one candidate displacement (Dx, Dy) px, all threads of the
+// workgroup: samples every 8 pixels (every other position of the 1/4-resolution
+// grid, src/encoder.c:184-185); a sample displaced off the frame costs 255*4
+// (src/encoder.c:190).  The result is uniform.
+__device__ int64_t gmv_direct(const PicDev &C, const PicDev &R, int nx, int ns, int Dx, int Dy,
+                              int *red)
+{
+    const gptr8 cy = (gptr8)C.y, ry = (gptr8)R.y;
+    int s = 0;
+    for (int k = threadIdx.x; k < ns; k += kGmvThreads) {
+        const int j = k / nx, i = k - j * nx;
+        const int rx = 8 * i + Dx, ry_ = 8 * j + Dy;
+        if (rx < 0 || ry_ < 0 || rx >= C.w || ry_ >= C.h) { s += 255 * 4; continue; }
+        const int o = cy[static_cast<size_t>(8 * j) * C.pitch + 8 * i];
+        const int r = ry[static_cast<size_t>(ry_) * R.pitch + rx];
+        s += o > r ? o - r : r - o;
+    }
+    return block_reduce(s, red);
+}
+
+__device__ __forceinline__ int gmv_check(const PicDev *table, int npics, int cp, int rp,
+                                         PicDev &C, PicDev &R)
+{
+    if (cp < 0 || cp >= npics || rp < 0 || rp >= npics) return TCME_E_PIC;
+    C = table[cp];
+    R = table[rp];
+    if (!C.y || !R.y) return TCME_E_PIC;
+    if (C.w != R.w || C.h != R.h) return TCME_E_ARG;
+    return 0;
+}
+
+// Every displacement the walk can visit is a multiple of 4 px (pass 0 moves
+// by 8, pass 1 by 4, from 0), so the coarse SADs of all displacements within
+// +-kGmvReach px are computed up front, in parallel: entry (iy, ix) holds
+// (Dx, Dy) = (4 ix - kGmvReach, 4 iy - kGmvReach).  Workgroup (chunk, band,
+// pair) takes 256 entries and the band's `rows` sample rows j (picture row
+// 8j), stages the band's current samples and the reference rows/columns at
+// 4-px spacing those entries read, and writes one partial sum per entry.
+// A thread owns one entry, so lanes read neighbouring reference bytes.
+__global__ __launch_bounds__(256) void gmv_table_kernel(const PicDev *__restrict__ table,
+                                                        int npics,
+                                                        const int32_t *__restrict__ pairs,
+                                                        int rows, int nbands,
+                                                        int32_t *__restrict__ part)
+{
+    extern __shared__ uint8_t gmv_lds[];
+    const int pair = blockIdx.z, band = blockIdx.y;
+    PicDev C{}, R{};
+    if (gmv_check(table, npics, pairs[2 * pair], pairs[2 * pair + 1], C, R)) return;
+    const int w = C.w, h = C.h, nx = (w / 4 + 1) / 2, ny = (h / 4 + 1) / 2;
+    const int j0 = band * rows;
+    if (j0 >= ny) return;
+    const int nj = min(rows, ny - j0);
+    const int e0 = blockIdx.x * 256, e = e0 + threadIdx.x;
+    const int iy_lo = e0 / kGmvSide, iy_hi = min(e0 + 255, kGmvEntries - 1) / kGmvSide;
+    // staged reference rows: picture row 4*r4, r4 = 2j + iy - kGmvReach/4
+    const int q = kGmvReach / 4;
+    const int r4_base = 2 * j0 + iy_lo - q;
+    const int nr = 2 * (nj - 1) + (iy_hi - iy_lo) + 1;
+    const int lc = 2 * nx + 2 * q + 1;  // columns c4 = -q .. 2(nx-1)+q, picture column 4*c4
+    uint8_t *ref_s = gmv_lds;
+    uint8_t *cur_s = gmv_lds + nr * lc;
+    const gptr8 cy = (gptr8)C.y, ry = (gptr8)R.y;
+    for (int k = threadIdx.x; k < nr * lc; k += 256) {
+        const int r = k / lc, c4 = k - r * lc - q, r4 = r4_base + r;
+        const bool in = r4 >= 0 && 4 * r4 < h && c4 >= 0 && 4 * c4 < w;
+        ref_s[k] = in ? ry[static_cast<size_t>(4 * r4) * R.pitch + 4 * c4] : 0;
+    }
+    for (int k = threadIdx.x; k < nj * nx; k += 256) {
+        const int jj = k / nx, i = k - jj * nx;
+        cur_s[k] = cy[static_cast<size_t>(8 * (j0 + jj)) * C.pitch + 8 * i];
+    }
+    __syncthreads();
+    if (e >= kGmvEntries) return;
+    const int iy = e / kGmvSide, ix = e - iy * kGmvSide;
+    const unsigned cw4 = static_cast<unsigned>((w + 3) / 4);  // c4 in frame: 0 <= c4 < cw4
+    int acc = 0;
+    for (int jj = 0; jj < nj; ++jj) {
+        const int r4 = 2 * (j0 + jj) + iy - q;
+        if (r4 < 0 || 4 * r4 >= h) { acc += nx * (255 * 4); continue; }
+        const uint8_t *rr = ref_s + (r4 - r4_base) * lc + ix;  // column c4 = 2i + ix - q
+        const uint8_t *cr = cur_s + jj * nx;
+        for (int i = 0; i < nx; ++i) {
+            const int d = abs(static_cast<int>(cr[i]) - static_cast<int>(rr[2 * i]));
+            acc += static_cast<unsigned>(2 * i + ix - q) < cw4 ? d : 255 * 4;
+        }
+    }
+    part[(static_cast<size_t>(pair) * nbands + band) * kGmvEntries + e] = acc;
+}
+
+// One workgroup per (current, reference) pair: sums the bands of the table,
+// then walks as the reference does, candidates in row-major order around the
+// moving best, the first of each pass always replacing the initial 2^60
+// bound (src/encoder.c:178-197).  A displacement outside the table is priced
+// directly.
+__global__ __launch_bounds__(kGmvThreads) void gmv_walk_kernel(const PicDev *__restrict__ table,
+                                                               int npics,
+                                                               const int32_t *__restrict__ pairs,
+                                                               int n, int rows, int nbands,
+                                                               const int32_t *__restrict__ part,
+                                                               int32_t *__restrict__ out)
 {
     __shared__ int red[kGmvThreads / 64];
+    __shared__ int64_t tab[kGmvEntries];
     const int i = blockIdx.x;
     if (i >= n) return;
-    const int cp = pairs[2 * i], rp = pairs[2 * i + 1];
-    int status = 0;
     PicDev C{}, R{};
-    if (cp < 0 || cp >= npics || rp < 0 || rp >= npics) status = TCME_E_PIC;
-    else {
-        C = table[cp];
-        R = table[rp];
-        if (!C.y || !R.y) status = TCME_E_PIC;
-        else if (C.w != R.w || C.h != R.h) status = TCME_E_ARG;
-    }
+    const int status = gmv_check(table, npics, pairs[2 * i], pairs[2 * i + 1], C, R);
     if (status) {
         if (threadIdx.x == 0) { out[3 * i] = 0; out[3 * i + 1] = 0; out[3 * i + 2] = status; }
         return;
     }
-    const int w = C.w, h = C.h, cw = w / 4, ch = h / 4;
-    const int nx = (cw + 1) / 2, ny = (ch + 1) / 2;   // x, y = 0, 2, 4, .. < cw, ch
-    const int ns = nx * ny;
-    const gptr8 cy = (gptr8)C.y, ry = (gptr8)R.y;
+    const int nx = (C.w / 4 + 1) / 2, ny = (C.h / 4 + 1) / 2, ns = nx * ny;
+    const int nb = (ny + rows - 1) / rows;
+    for (int e = threadIdx.x; e < kGmvEntries; e += kGmvThreads) {
+        int64_t t = 0;
+        for (int b = 0; b < nb; ++b) t += part[(static_cast<size_t>(i) * nbands + b) * kGmvEntries + e];
+        tab[e] = t;
+    }
+    __syncthreads();
     int bx = 0, by = 0;
     int64_t best = static_cast<int64_t>(1) << 60;
     for (int pass = 0; pass < 2; ++pass) {
@@ -74,20 +170,15 @@ __global__ __launch_bounds__(kGmvThreads) void global_mv_kernel(const PicDev *__
         bool improved = false;
         for (int dy = -range; dy <= range; dy += step)
             for (int dx = -range; dx <= range; dx += step) {
-                int s = 0;
-                for (int k = threadIdx.x; k < ns; k += kGmvThreads) {
-                    const int y = 2 * (k / nx), x = 2 * (k - (k / nx) * nx);
-                    const int rx = x * 4 + bx + dx * 4, ry_ = y * 4 + by + dy * 4;
-                    if (rx < 0 || ry_ < 0 || rx >= w || ry_ >= h) { s += 255 * 4; continue; }
-                    const int o = cy[static_cast<size_t>(y * 4) * C.pitch + x * 4];
-                    const int r = ry[static_cast<size_t>(ry_) * R.pitch + rx];
-                    s += o > r ? o - r : r - o;
-                }
-                const int64_t tot = block_reduce(s, red);
+                const int Dx = bx + dx * 4, Dy = by + dy * 4;
+                const int64_t tot =
+                    (Dx >= -kGmvReach && Dx <= kGmvReach && Dy >= -kGmvReach && Dy <= kGmvReach)
+                        ? tab[((Dy + kGmvReach) >> 2) * kGmvSide + ((Dx + kGmvReach) >> 2)]
+                        : gmv_direct(C, R, nx, ns, Dx, Dy, red);
                 if (tot < best) {
                     best = tot;
-                    bx += dx * 4;
-                    by += dy * 4;
+                    bx = Dx;
+                    by = Dy;
                     improved = true;
                 }
             }
@@ -153,12 +244,37 @@ __global__ __launch_bounds__(256) void sad_kernel(const PicDev *__restrict__ tab
 
 }  // namespace
 
+int gmv_rows(int max_w)
+{
+    const int nx = (max_w / 4 + 1) / 2;
+    for (int rows = 16; rows > 1; rows >>= 1) {
+        const int nr = 2 * (rows - 1) + 5, lc = 2 * nx + kGmvSide;
+        if (nr * lc + rows * nx <= kGmvLdsMax) return rows;
+    }
+    return 1;
+}
+
+size_t gmv_part_bytes(int n, int max_w, int max_h)
+{
+    const int rows = gmv_rows(max_w), ny = (max_h / 4 + 1) / 2;
+    const int nbands = std::max(1, (ny + rows - 1) / rows);
+    return static_cast<size_t>(n) * nbands * kGmvEntries * sizeof(int32_t);
+}
+
 hipError_t launch_global_mv(const PicDev *table, int npics, const int32_t *pairs, int n,
-                            int32_t *out, hipStream_t s)
+                            int max_w, int max_h, int32_t *part, int32_t *out, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(global_mv_kernel, dim3(n), dim3(kGmvThreads), 0, s, table, npics, pairs, n,
-                       out);
+    const int rows = gmv_rows(max_w), nx = (max_w / 4 + 1) / 2, ny = (max_h / 4 + 1) / 2;
+    const int nbands = std::max(1, (ny + rows - 1) / rows);
+    const int nr = 2 * (rows - 1) + 5, lc = 2 * nx + kGmvSide;
+    const size_t lds = static_cast<size_t>(nr) * lc + static_cast<size_t>(rows) * nx;
+    const int chunks = (kGmvEntries + 255) / 256;
+    hipLaunchKernelGGL(gmv_table_kernel, dim3(chunks, nbands, n), dim3(256), lds, s, table, npics,
+                       pairs, rows, nbands, part);
+    if (hipError_t e = hipGetLastError()) return e;
+    hipLaunchKernelGGL(gmv_walk_kernel, dim3(n), dim3(kGmvThreads), 0, s, table, npics, pairs, n,
+                       rows, nbands, static_cast<const int32_t *>(part), out);
     return hipGetLastError();
 }
 

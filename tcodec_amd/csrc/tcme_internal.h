@@ -38,9 +38,12 @@ struct Tiling {
 };
 
 bool valid_waves(int blk, int waves);
-// Global-motion estimates and block SADs (tcme_aux.hip).
+// Global-motion estimates and block SADs (tcme_aux.hip).  The estimate
+// needs gmv_part_bytes(n, W, H) bytes of workspace, W x H bounding every
+// picture a pair may name.
+size_t gmv_part_bytes(int n, int max_w, int max_h);
 hipError_t launch_global_mv(const PicDev *table, int npics, const int32_t *pairs, int n,
-                            int32_t *out, hipStream_t s);
+                            int max_w, int max_h, int32_t *part, int32_t *out, hipStream_t s);
 hipError_t launch_sad(const PicDev *table, int npics, const tcme_sad_desc *descs, int n,
                       const uint8_t *buf, int64_t buf_bytes, int32_t *out, hipStream_t s);
 // Motion compensation of n blocks (tcme_mc.hip): luma from the anchor plane,

@@ -29,7 +29,8 @@ EXPORTS = (
     "tcme_pic_luma", "tcme_pic_finish", "tcme_search", "tcme_search_host",
     "tcme_set_tiling", "tcme_get_tiling", "tcme_autotune", "tcme_set_timing",
     "tcme_last_timing", "tcme_pic_create_plane", "tcme_predict_luma", "tcme_predict_chroma",
-    "tcme_global_mv", "tcme_sad",
+    "tcme_global_mv", "tcme_sad", "tcme_graph_create", "tcme_graph_launch",
+    "tcme_graph_destroy",
 )
 
 CALL_DTYPE = np.dtype([("ref_pic", "<i4"), ("cur_pic", "<i4"), ("px", "<i4"), ("py", "<i4"),
@@ -130,6 +131,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
                       C.c_void_p]
     L.tcme_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.c_int]
+    L.tcme_graph_create.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.POINTER(C.c_void_p)]
+    L.tcme_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
+    L.tcme_graph_destroy.argtypes = [C.c_void_p]
+    L.tcme_graph_destroy.restype = None
     _lib = L
     return L
 
@@ -310,6 +316,30 @@ class Engine:
         _check(self.L.tcme_search(self.h, C.c_void_p(calls_ptr), n, C.c_void_p(res_ptr),
                                   C.c_void_p(stats_ptr) if stats_ptr else None,
                                   C.c_void_p(stream)))
+
+    def graph(self, calls_ptr: int, n: int, res_ptr: int,
+              stats_ptr: int | None = None) -> "Graph":
+        """The search of this device call list captured as a HIP graph."""
+        g = C.c_void_p()
+        _check(self.L.tcme_graph_create(self.h, C.c_void_p(calls_ptr), n, C.c_void_p(res_ptr),
+                                        C.c_void_p(stats_ptr) if stats_ptr else None,
+                                        C.byref(g)))
+        return Graph(self, g)
+
+
+class Graph:
+    """A captured search (tcme_graph_*): replay with ``launch(stream)``."""
+
+    def __init__(self, eng: Engine, handle: C.c_void_p):
+        self.eng, self.h = eng, handle
+
+    def launch(self, stream: int = 0) -> None:
+        _check(self.eng.L.tcme_graph_launch(self.h, C.c_void_p(stream)))
+
+    def close(self) -> None:
+        if self.h:
+            self.eng.L.tcme_graph_destroy(self.h)
+            self.h = None
 
 
 def make_calls(ref_pic, cur_pic, px, py, cx, cy, blk, sr) -> np.ndarray:

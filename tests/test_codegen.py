@@ -58,4 +58,4 @@ def test_every_kernel_present():
         assert re.search(rf"me_tile_kernelILi{w}E", text), w
     assert "anchor_kernel" in text
     assert "mc_luma_kernel" in text and "mc_chroma_kernel" in text
-    assert "global_mv_kernel" in text and "sad_kernel" in text
+    assert "gmv_table_kernel" in text and "gmv_walk_kernel" in text and "sad_kernel" in text

@@ -26,6 +26,20 @@ def test_gpu_global_mv_matches_oracle(engine, orc):
     planes += [rp, cp]
     pairs.append((cp, rp))
     want.append(orc.global_mv(f3, f0))
+    # A long ramp (triangle wave, period 1024 px, shifted 250 px) on a wide
+    # picture: the walk travels 224 px, past the +-128 px displacement table,
+    # so candidates beyond it are priced directly.
+    x = np.arange(4096)
+    f = np.mod(x, 1024) / 1024
+    tri = lambda v: np.round(np.where(v < 0.5, v * 2, 2 - v * 2) * 255).astype(np.uint8)
+    g = np.mod(x + 250, 1024) / 1024
+    cur = np.ascontiguousarray(np.tile(tri(f), (256, 1)))
+    ref = np.ascontiguousarray(np.tile(tri(g), (256, 1)))
+    rp, cp = engine.pic(ref), engine.pic(cur)
+    planes += [rp, cp]
+    pairs.append((cp, rp))
+    want.append(orc.global_mv(cur, ref))
+    assert want[-1] == (-768, 0)
     got = engine.global_mv(pairs)
     assert (got[:, 2] == 0).all()
     assert [tuple(g[:2]) for g in got] == [tuple(x) for x in want]

@@ -286,3 +286,46 @@ def test_gpu_kernel_timing(engine):
         engine.set_tiling(**old.as_dict())
         engine.free(rp)
         engine.free(cp)
+
+
+def test_gpu_graph_replay(engine):
+    """tcme_graph_*: the captured search replayed several times, the call list
+    rewritten between replays and eager searches with synchronisations in
+    between; every replay equals the eager result of the list it read, for the
+    split launch (fork/join over side streams) and for tile_big."""
+    import torch
+    from tcodec_amd import me, synth, workload
+    fr = synth.scene_video(256, 192, 4, seed=9, rects=4)
+    pics = [engine.pic(f[0]) for f in fr]
+    lists = [workload.frame_calls_by_ctu(256, 192, [pics[0], pics[1]], pics[2], workload.SR_B),
+             workload.frame_calls_by_ctu(256, 192, [pics[1], pics[3]], pics[2], workload.SR_B),
+             workload.frame_calls_by_ctu(256, 192, [pics[3], pics[0]], pics[1], workload.SR_B)]
+    n = len(lists[0])
+    assert all(len(x) == n for x in lists)
+    want = [engine.search(x) for x in lists]
+    host = [torch.from_numpy(x.view(np.int32).reshape(-1, 8).copy()) for x in lists]
+    calls = host[0].cuda()
+    res = torch.zeros((n, 4), dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    old = engine.tiling()
+    try:
+        for tiling in ((1, 1, 1, 4, 8, 85, 0), (1, 1, 1, 4, 4, 85, 1)):
+            log(f"graph replay tiling {tiling}")
+            engine.set_tiling(*tiling)
+            g = engine.graph(calls.data_ptr(), n, res.data_ptr())
+            try:
+                for rep, k in enumerate((0, 1, 2, 1, 0, 2)):
+                    calls.copy_(host[k])
+                    res.fill_(-7)
+                    g.launch(stream.cuda_stream)
+                    torch.cuda.synchronize()
+                    got = res.cpu().numpy().view(me.RESULT_DTYPE).ravel()
+                    assert (got == want[k]).all(), (tiling, rep, k)
+                    # eager work between replays
+                    assert (engine.search(lists[(k + 1) % 3]) == want[(k + 1) % 3]).all()
+            finally:
+                g.close()
+    finally:
+        engine.set_tiling(**old.as_dict())
+        for p in pics:
+            engine.free(p)
