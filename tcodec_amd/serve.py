@@ -140,9 +140,11 @@ class Stream:
             eng.free(p)
         self.ring = []
 
-    def prepare(self, eng: me.Engine, tpl: Templates, stream_ptr: int):
+    def prepare(self, eng: me.Engine, tpl: Templates, stream_ptr: int, trace=None):
         """Loads the next coded picture into the ring and returns its call
-        list on the device (None for an intra picture).  Advances pos."""
+        list on the device (None for an intra picture).  Advances pos.  With a
+        trace.TraceWriter, the picture is recorded as a plane (its device
+        picture id is the plane id)."""
         poc, kind, dpb = self.order[self.pos]
         self.pos += 1
         keep = set(dpb)
@@ -152,6 +154,11 @@ class Stream:
         self.slot_of[poc] = slot
         eng.upload_device(self.ring[slot], self.clip.plane_ptr(self.offset + poc), self.w,
                           stream_ptr)
+        if trace is not None:
+            t = (self.offset + poc) % self.clip.frames
+            host = self.clip.host[t] if self.clip.host is not None else \
+                self.clip.dev[t].cpu().numpy()
+            trace.plane(self.ring[slot], host)
         if kind == "I" or not dpb:
             return None
         sr = workload.SR_B if kind == "B" else workload.SR_P
@@ -168,8 +175,9 @@ class Server:
     """Continuous-batching request loop over one tcme context."""
 
     def __init__(self, eng: me.Engine, hbm_budget: int, max_tick_calls: int = 6_000_000,
-                 keep_results: bool = False, per_stream_launch: bool = False):
+                 keep_results: bool = False, per_stream_launch: bool = False, trace=None):
         self.eng = eng
+        self.trace = trace        # trace.TraceWriter: record planes and searches
         self.per_stream = per_stream_launch
         self.budget = hbm_budget
         self.max_tick_calls = max_tick_calls
@@ -218,7 +226,7 @@ class Server:
         """One launch over the next picture of every active stream."""
         lists, owners = [], []
         for s in self.active:
-            c = s.prepare(self.eng, self.tpl, self.sptr)
+            c = s.prepare(self.eng, self.tpl, self.sptr, self.trace)
             if c is not None:
                 lists.append(c)
                 owners.append((s, c.shape[0]))
@@ -229,7 +237,7 @@ class Server:
                 r = torch.empty((c.shape[0], 4), dtype=torch.int32, device="cuda")
                 self.eng.search_device(c.data_ptr(), c.shape[0], r.data_ptr(), None, self.sptr)
                 outs.append(r)
-            res = torch.cat(outs) if self.keep else None
+            res = torch.cat(outs) if (self.keep or self.trace is not None) else None
         elif lists:
             calls = torch.cat(lists) if len(lists) > 1 else lists[0]
             res = torch.empty((calls.shape[0], 4), dtype=torch.int32, device="cuda")
@@ -237,6 +245,9 @@ class Server:
                                    self.sptr)
         self.stream.synchronize()
         end = time.perf_counter()
+        if lists and self.trace is not None:
+            from .trace import rows_from
+            self.trace.calls(rows_from(torch.cat(lists).cpu().numpy(), res.cpu().numpy()))
         if lists and self.keep:
             host = res.cpu().numpy()
             o = 0

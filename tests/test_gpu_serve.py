@@ -63,3 +63,31 @@ def test_batched_one_launch_equals_per_picture(engine):
         assert len(a[sid]) == len(b[sid]) == 6
         for x, y in zip(a[sid], b[sid]):
             assert np.array_equal(x, y), sid
+
+
+def test_gpu_trace_replays(engine, tmp_path):
+    """TCMT traces replayed on the GPU: the golden traces of the reference
+    encoder, and a serving session recorded by the server itself (streams
+    admitted and released, so picture ids are reused as trace slots)."""
+    import io
+
+    from helpers import golden_traces
+    from tcodec_amd import serve, trace
+    from test_trace import to_tcmt
+    print("trace replay: golden", flush=True)
+    for name, refs, curs, calls in golden_traces():
+        data, _ = to_tcmt(refs, curs, calls)
+        r = trace.replay(engine, io.BytesIO(data))
+        assert r["calls"] == calls.shape[0] and r["mismatches"] == 0, (name, r)
+    print("trace replay: served session", flush=True)
+    clip_a = serve.Clip(160, 128, 5, seed=21)
+    clip_b = serve.DeviceGradientClip(256, 144, 6, seed=22)
+    path = str(tmp_path / "served.tcmt")
+    with trace.TraceWriter(path) as tw:
+        srv = serve.Server(engine, hbm_budget=2 * serve.stream_bytes(256, 144), trace=tw)
+        srv.submit(serve.Stream(0, 160, 128, 5, "B", clip_a, offset=1, arrival=0.0))
+        srv.submit(serve.Stream(1, 256, 144, 6, "P", clip_b, offset=0, arrival=0.0))
+        srv.submit(serve.Stream(2, 160, 128, 5, "P", clip_a, offset=2, arrival=0.01))
+        srv.run()
+    r = trace.replay(engine, path)
+    assert r["calls"] == srv.calls and r["calls"] > 0 and r["mismatches"] == 0, r
