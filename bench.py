@@ -56,7 +56,7 @@ def build_workload(eng):
 def kernels(tiling: dict):
     """[(kernel name, block sizes it searches)] in the library's launch order."""
     if tiling["tile_waves"] and tiling["tile_big"]:
-        return [("me_tile_kernel<4,big>", (8, 16, 32, 64))]
+        return [("me_tile_kernel<%d,big>" % tiling["tile_waves"], (8, 16, 32, 64))]
     out = [("me_search_kernel<64,%d>" % tiling["waves64"], (64,)),
            ("me_search_kernel<32,%d>" % tiling["waves32"], (32,))]
     if tiling["tile_waves"]:

@@ -116,8 +116,8 @@ int  tcme_search(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *resu
  *                    many calls per wavefront from there.  Lists grouped by
  *                    CTU and reference (85 calls per group) suit chunk 85.
  *                    0: no tile workgroups.
- *   tile_big         1 (with tile_waves 4): the tile workgroups take every
- *                    call, all block sizes */
+ *   tile_big         1 (with tile_waves 4 or 8): the tile workgroups take
+ *                    every call, all block sizes (one launch) */
 typedef struct tcme_tiling {
     int32_t waves8, waves16, waves32, waves64;
     int32_t tile_waves, tile_chunk;

@@ -183,8 +183,8 @@ int dispatch(tcme_ctx *c, const tcme_call *calls, int n, const tcme::Tiling &t,
     if (t.tile_waves && t.tile_big) {
         // one launch: the tile kernel takes every call
         if (c->timing) HIP_TRY(hipEventRecord(c->t0[0], s));
-        HIP_TRY(tcme::launch_tile(4, true, t.tile_chunk, c->d_table, c->max_pics, calls, n, res,
-                                  stats, s));
+        HIP_TRY(tcme::launch_tile(t.tile_waves, true, t.tile_chunk, c->d_table, c->max_pics, calls,
+                                  n, res, stats, s));
         if (c->timing) HIP_TRY(hipEventRecord(c->t1[0], s));
         c->nlaunched = 1;
         return TCME_OK;
@@ -366,8 +366,8 @@ int check_tiling(const tcme_tiling *t)
         return fail(TCME_E_ARG, "%d waves per tile workgroup not supported", t->tile_waves);
     if (t->tile_waves && (t->tile_chunk < 1 || t->tile_chunk > tcme::kTileChunkMax))
         return fail(TCME_E_ARG, "tile chunk %d outside 1..%d", t->tile_chunk, tcme::kTileChunkMax);
-    if (t->tile_big != 0 && !(t->tile_big == 1 && t->tile_waves == 4))
-        return fail(TCME_E_ARG, "tile_big needs tile_waves 4");
+    if (t->tile_big != 0 && !(t->tile_big == 1 && (t->tile_waves == 4 || t->tile_waves == 8)))
+        return fail(TCME_E_ARG, "tile_big needs tile_waves 4 or 8");
     return TCME_OK;
 }
 
@@ -423,7 +423,7 @@ TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_resu
     hipStream_t s = static_cast<hipStream_t>(stream);
     static const int w32[] = {1, 2, 4};
     static const int w64[] = {2, 4, 8};
-    static const int tw[] = {0, 4, 8, -4};  // -4: tile_big with 4 wavefronts
+    static const int tw[] = {0, 4, 8, -4, -8};  // negative: tile_big with that many wavefronts
     const int chunk = c->tiling.tile_chunk;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));

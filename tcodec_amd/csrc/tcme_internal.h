@@ -34,7 +34,7 @@ struct Tiling {
     int waves[4] = {1, 1, 1, 4};  // for block sizes 8, 16, 32, 64
     int tile_waves = 8;
     int tile_chunk = 85;
-    int tile_big = 0;             // tile workgroups also take 32x32/64x64 (4 wavefronts)
+    int tile_big = 1;             // tile workgroups also take 32x32/64x64: one launch
 };
 
 bool valid_waves(int blk, int waves);

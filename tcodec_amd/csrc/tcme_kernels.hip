@@ -848,7 +848,9 @@ struct GroupWalk {
 
     // One pattern stage, exactly the reference's loop: candidates in table
     // order around the current best, a strictly better one moves the centre
-    // at once; a pass without a move ends the stage.
+    // at once; a pass without a move ends the stage.  (All the group's calls
+    // step through the candidates together, so a pass costs K candidate
+    // prices per wavefront whatever the calls do.)
     template <int P>
     __device__ __forceinline__ void stage(int scale, int passes, int &bx, int &by, int &best,
                                           int &ie, int &oob) const
@@ -1078,6 +1080,40 @@ __device__ __forceinline__ void run_deferred(TileShared<WAVES> &sh, const PicDev
     }
 }
 
+// Quarter-pel pass of deferred 64x64 call d by the whole workgroup (a 64x64
+// block held by one wavefront would need ~170 registers; spread over WAVES
+// wavefronts each lane holds 64/WAVES samples).  qw: the 66x66 anchor window,
+// red: 2 * WAVES * 8 ints for the cross-wavefront sums.  Every wavefront of
+// the workgroup calls this with the same d.
+template <int WAVES>
+__device__ __forceinline__ void run_deferred_wg(TileShared<WAVES> &sh,
+                                                const PicDev *__restrict__ table, const PicDev &R,
+                                                int d, anc_t *qw, int *red,
+                                                tcme_result *__restrict__ res,
+                                                tcme_stats *__restrict__ stats)
+{
+    const int ci = sh.def_ci[0][d];
+    const tcme_call c = sh.call[ci];
+    const PicDev C = table[c.cur_pic];
+    Searcher<64, WAVES> Q(R, C, c.px, c.py, qw, red);
+    int32_t best = sh.def_sad[0][d];
+    int qx, qy;
+    Q.qpel(sh.def_x[0][d], sh.def_y[0][d], qx, qy, best);
+    if (Q.tid == 0) {
+        const int i = sh.idx[ci];
+        res[i].mv_x = qx;
+        res[i].mv_y = qy;
+        res[i].sad = best;
+        res[i].status = 0;
+        if (stats) {
+            stats[i].int_evals = sh.def_ie[0][d];
+            stats[i].q6_evals = Q.cnt.q6;
+            stats[i].qbil_evals = Q.cnt.qb;
+            stats[i].int_oob = sh.def_oob[0][d];
+        }
+    }
+}
+
 // Every lane adds 1, so a fetch advances the counter by 64 and the item index
 // is the first lane's old value / 64.  (A fetch under `if (lane == 0)`
 // followed by a broadcast lets the compiler run the other lanes through the
@@ -1091,7 +1127,7 @@ __device__ __forceinline__ int fetch_item(int *counter)
 // BIG: the workgroups also take the 32x32 and 64x64 calls (else those are
 // left to me_search_kernel, and calls of unsupported sizes to its 64 instance).
 template <int WAVES, bool BIG>
-__global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__restrict__ table,
+__global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile_kernel(const PicDev *__restrict__ table,
                                                             int npics,
                                                             const tcme_call *__restrict__ calls,
                                                             int n, int chunk,
@@ -1246,12 +1282,17 @@ __global__ __launch_bounds__(64 * WAVES) void me_tile_kernel(const PicDev *__res
         const int d64 = sh.ndef[0], d32 = sh.ndef[1], d16 = sh.ndef[2];
         if (d64 + d32 + d16 > 0) {
             constexpr int W32 = 34 * 34, W16 = 18 * 18;
-            constexpr int P32 = WAVES < 5 ? WAVES : 5;
+            constexpr int kQw = kLatWords / 2;  // anchors the union holds
+            constexpr int P32 = WAVES < kQw / W32 ? WAVES : kQw / W32;
+            static_assert(66 * 66 + (2 * WAVES * 8 + 1) / 2 <= kQw, "64x64 window + sums");
+            static_assert(P32 >= 1 && WAVES * W16 <= kQw, "deferred windows");
             if constexpr (BIG) {
-                // one 64x64 window fits: wavefront 0 takes them all
-                if (wave == 0)
-                    for (int s = 0; s < d64; ++s)
-                        run_deferred<64>(sh, table, R, 0, s, sh.qw, lane, res, stats);
+                // one 64x64 window fits: the whole workgroup refines them in turn
+                for (int s = 0; s < d64; ++s) {
+                    run_deferred_wg<WAVES>(sh, table, R, s, sh.qw,
+                                           reinterpret_cast<int *>(sh.qw + 66 * 66), res, stats);
+                    __syncthreads();
+                }
                 if (threadIdx.x == 0) sh.next = 0;
                 __syncthreads();
                 if (wave < P32) {
@@ -1351,8 +1392,11 @@ hipError_t launch_tile(int waves, bool big, int chunk, const PicDev *table, int 
 {
     if (n <= 0) return hipSuccess;
     const int grid = (n + chunk - 1) / chunk;
-    if (big)
+    if (big && waves == 4)
         hipLaunchKernelGGL((me_tile_kernel<4, true>), dim3(grid), dim3(256), 0, s, table, npics,
+                           calls, n, chunk, res, stats);
+    else if (big)
+        hipLaunchKernelGGL((me_tile_kernel<8, true>), dim3(grid), dim3(512), 0, s, table, npics,
                            calls, n, chunk, res, stats);
     else if (waves == 4)
         hipLaunchKernelGGL((me_tile_kernel<4, false>), dim3(grid), dim3(256), 0, s, table, npics,

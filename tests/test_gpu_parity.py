@@ -132,7 +132,9 @@ def test_gpu_rejects_bad_tiling(engine):
     with pytest.raises(me.TcmeError):
         engine.set_tiling(1, 1, 2, 4, 4, 97)
     with pytest.raises(me.TcmeError):
-        engine.set_tiling(1, 1, 2, 4, 8, 85, 1)
+        engine.set_tiling(1, 1, 2, 4, 0, 85, 1)    # tile_big without tile workgroups
+    with pytest.raises(me.TcmeError):
+        engine.set_tiling(1, 1, 2, 4, 8, 85, 2)
 
 
 def test_gpu_rejects_bad_descriptors(engine):
@@ -198,7 +200,8 @@ def test_dropin_symbol_matches_oracle(orc):
 
 TILINGS = [(1, 1, 1, 2, 0, 85, 0), (1, 1, 2, 4, 4, 85, 0), (1, 1, 4, 8, 8, 85, 0),
            (1, 1, 1, 4, 8, 1, 0), (1, 1, 1, 4, 8, 37, 0), (1, 1, 1, 4, 4, 96, 0),
-           (1, 1, 1, 4, 4, 85, 1), (1, 1, 1, 4, 4, 7, 1)]
+           (1, 1, 1, 4, 4, 85, 1), (1, 1, 1, 4, 4, 7, 1), (1, 1, 1, 4, 8, 85, 1),
+           (1, 1, 1, 4, 8, 23, 1)]
 
 
 @pytest.mark.parametrize("tiling", TILINGS, ids=lambda t: "w%d_%d_%d_%d_t%d_c%d_b%d" % t)
