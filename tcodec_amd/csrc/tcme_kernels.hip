@@ -988,6 +988,7 @@ struct TileShared {
     int ndef[3];
     int nlive;
     int next;                        // work counter, 64 per item taken
+    int next32;                      // the same, deferred 32x32 calls
 };
 
 template <int N, int WAVES>
@@ -1277,8 +1278,10 @@ __global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile
         }
         __syncthreads();
         // Deferred quarter-pel passes; the lattice's LDS now holds anchor
-        // windows: (64+2)^2 for one 64x64 call, (32+2)^2 each for up to five
-        // 32x32 calls, (16+2)^2 for every wavefront's 8x8/16x16 call.
+        // windows: first (64+2)^2 for the 64x64 calls, refined one after the
+        // other by the whole workgroup; then (16+2)^2 for every wavefront's
+        // 8x8/16x16 call and, after those, (32+2)^2 for each of the first few
+        // wavefronts' 32x32 call, both kinds refined at the same time.
         const int d64 = sh.ndef[0], d32 = sh.ndef[1], d16 = sh.ndef[2];
         if (d64 + d32 + d16 > 0) {
             constexpr int W32 = 34 * 34, W16 = 18 * 18;
@@ -1293,19 +1296,25 @@ __global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile
                                            reinterpret_cast<int *>(sh.qw + 66 * 66), res, stats);
                     __syncthreads();
                 }
-                if (threadIdx.x == 0) sh.next = 0;
+                // 32x32 and small calls together: wavefronts below Q32 take the
+                // 32x32 ones (windows after every wavefront's small window),
+                // then join the others on the small ones
+                constexpr int Q32 = (kQw - WAVES * W16) / W32 < P32 ? (kQw - WAVES * W16) / W32 : P32;
+                static_assert(Q32 >= 1, "a 32x32 window beside the small ones");
+                if (threadIdx.x == 0) { sh.next = 0; sh.next32 = 0; }
                 __syncthreads();
-                if (wave < P32) {
+                if (wave < Q32) {
                     for (;;) {
-                        const int s = fetch_item(&sh.next);
+                        const int s = fetch_item(&sh.next32);
                         if (s >= d32) break;
-                        run_deferred<32>(sh, table, R, 1, s, sh.qw + wave * W32, lane, res, stats);
+                        run_deferred<32>(sh, table, R, 1, s, sh.qw + WAVES * W16 + wave * W32, lane,
+                                         res, stats);
                     }
                 }
+            } else {
+                if (threadIdx.x == 0) sh.next = 0;
                 __syncthreads();
             }
-            if (threadIdx.x == 0) sh.next = 0;
-            __syncthreads();
             for (;;) {
                 const int s = fetch_item(&sh.next);
                 if (s >= d16) break;
