@@ -334,9 +334,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="arrival window of the served run (the serving config uses 120)")
     ap.add_argument("--serve-budget-gib", type=float, default=48.0)
-    ap.add_argument("--launch", choices=("graph", "direct"), default="graph",
-                    help="graph: per-frame searches captured as HIP graphs and replayed "
-                         "(the value); direct: one tcme_search call per frame")
+    ap.add_argument("--launch", choices=("graph", "direct"), default="direct",
+                    help="direct: one tcme_search call per frame (the value); graph: the "
+                         "per-frame searches captured as HIP graphs and replayed (the value); "
+                         "the other form is reported beside it")
     ap.add_argument("--mode", choices=("tuned", "default"), default="tuned",
                     help="default: the built-in launch shape only; tuned: also the autotuned "
                          "shape (reported as value)")
@@ -446,16 +447,20 @@ def main():
     direct = {"frames_per_s": round(nframes / elapsed, 2),
               "p50_ms": round(float(np.percentile(lat, 50)), 3),
               "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+    # each frame's search captured once (chosen shape), replayed per step
+    for i, (poc, kind, calls) in enumerate(frames):
+        graphs.append(eng.graph(d_calls[i].data_ptr(), len(calls), d_res[i].data_ptr()))
+    g_el, g_lat = timed("graph replay")
+    for g in graphs:
+        g.close()
+    graphs.clear()
+    for i in range(len(frames)):
+        assert (d_res[i][:, 3] == 0).all().item()
+    graph = {"frames_per_s": round(nframes / g_el, 2),
+             "p50_ms": round(float(np.percentile(g_lat, 50)), 3),
+             "p99_ms": round(float(np.percentile(g_lat, 99)), 3)}
     if args.launch == "graph":
-        # each frame's search captured once (chosen shape), replayed per step
-        for i, (poc, kind, calls) in enumerate(frames):
-            graphs.append(eng.graph(d_calls[i].data_ptr(), len(calls), d_res[i].data_ptr()))
-        elapsed, lat = timed("graph replay")
-        for g in graphs:
-            g.close()
-        graphs.clear()
-        for i in range(len(frames)):
-            assert (d_res[i][:, 3] == 0).all().item()
+        elapsed, lat = g_el, g_lat
     fps = nframes / elapsed
     calls_per_frame = int(np.mean([len(c) for _, _, c in frames]))
     # the dominant (longest) kernel: its own calls' algorithmic bytes over its
@@ -502,6 +507,7 @@ def main():
                        "load": "back-to-back per-frame launches, one stream"},
         "launch": args.launch,
         "direct_launch": direct,
+        "graph_launch": graph,
         "default_shape": {"value": round(nframes / plain_el, 2),
                           "p50_ms": round(float(np.percentile(plain_lat, 50)), 3)},
         "tiling": best.as_dict(),

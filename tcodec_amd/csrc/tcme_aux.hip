@@ -81,14 +81,20 @@ __device__ __forceinline__ int gmv_check(const PicDev *table, int npics, int cp,
 // pair) takes 256 entries and the band's `rows` sample rows j (picture row
 // 8j), stages the band's current samples and the reference rows/columns at
 // 4-px spacing those entries read, and writes one partial sum per entry.
-// A thread owns one entry, so lanes read neighbouring reference bytes.
+// A thread owns one entry.  Entry ix reads staged column 2i + ix for sample
+// i, so each staged reference row is kept as two halves (even columns, odd
+// columns): the reads of consecutive samples are then consecutive bytes of
+// one half, four samples per v_sad_u8.
+__host__ __device__ constexpr int gmv_half(int nx) { return ((nx + kGmvSide / 2 + 1 + 3) & ~3) + 4; }
+__host__ __device__ constexpr int gmv_curp(int nx) { return (nx + 3) & ~3; }
+
 __global__ __launch_bounds__(256) void gmv_table_kernel(const PicDev *__restrict__ table,
                                                         int npics,
                                                         const int32_t *__restrict__ pairs,
                                                         int rows, int nbands,
                                                         int32_t *__restrict__ part)
 {
-    extern __shared__ uint8_t gmv_lds[];
+    extern __shared__ uint32_t gmv_lds[];
     const int pair = blockIdx.z, band = blockIdx.y;
     PicDev C{}, R{};
     if (gmv_check(table, npics, pairs[2 * pair], pairs[2 * pair + 1], C, R)) return;
@@ -98,37 +104,61 @@ __global__ __launch_bounds__(256) void gmv_table_kernel(const PicDev *__restrict
     const int nj = min(rows, ny - j0);
     const int e0 = blockIdx.x * 256, e = e0 + threadIdx.x;
     const int iy_lo = e0 / kGmvSide, iy_hi = min(e0 + 255, kGmvEntries - 1) / kGmvSide;
-    // staged reference rows: picture row 4*r4, r4 = 2j + iy - kGmvReach/4
+    // staged reference rows: picture row 4*r4, r4 = 2j + iy - kGmvReach/4;
+    // staged column k = c4 + q (picture column 4*c4), k = 0 .. lc-1, row r's
+    // half p (= k & 1) at byte (2r + p) * hp + (k >> 1)
     const int q = kGmvReach / 4;
     const int r4_base = 2 * j0 + iy_lo - q;
     const int nr = 2 * (nj - 1) + (iy_hi - iy_lo) + 1;
-    const int lc = 2 * nx + 2 * q + 1;  // columns c4 = -q .. 2(nx-1)+q, picture column 4*c4
-    uint8_t *ref_s = gmv_lds;
-    uint8_t *cur_s = gmv_lds + nr * lc;
+    const int lc = 2 * nx + 2 * q + 1;
+    const int hp = gmv_half(nx), cp = gmv_curp(nx);
+    uint8_t *ref_s = reinterpret_cast<uint8_t *>(gmv_lds);
+    uint8_t *cur_s = ref_s + 2 * nr * hp;
     const gptr8 cy = (gptr8)C.y, ry = (gptr8)R.y;
-    for (int k = threadIdx.x; k < nr * lc; k += 256) {
-        const int r = k / lc, c4 = k - r * lc - q, r4 = r4_base + r;
-        const bool in = r4 >= 0 && 4 * r4 < h && c4 >= 0 && 4 * c4 < w;
+    for (int k = threadIdx.x; k < nr * 2 * hp; k += 256) {
+        const int rp = k / hp, m = k - rp * hp;
+        const int r = rp >> 1, c = 2 * m + (rp & 1);
+        const int c4 = c - q, r4 = r4_base + r;
+        const bool in = c < lc && r4 >= 0 && 4 * r4 < h && c4 >= 0 && 4 * c4 < w;
         ref_s[k] = in ? ry[static_cast<size_t>(4 * r4) * R.pitch + 4 * c4] : 0;
     }
-    for (int k = threadIdx.x; k < nj * nx; k += 256) {
-        const int jj = k / nx, i = k - jj * nx;
-        cur_s[k] = cy[static_cast<size_t>(8 * (j0 + jj)) * C.pitch + 8 * i];
+    for (int k = threadIdx.x; k < nj * cp; k += 256) {
+        const int jj = k / cp, i = k - jj * cp;
+        cur_s[k] = i < nx ? cy[static_cast<size_t>(8 * (j0 + jj)) * C.pitch + 8 * i] : 0;
     }
     __syncthreads();
     if (e >= kGmvEntries) return;
     const int iy = e / kGmvSide, ix = e - iy * kGmvSide;
-    const unsigned cw4 = static_cast<unsigned>((w + 3) / 4);  // c4 in frame: 0 <= c4 < cw4
+    const int cw4 = (w + 3) / 4;  // c4 in frame: 0 <= c4 < cw4
+    // samples i whose column 2i + ix - q is in frame: [ilo, ihi]
+    const int ilo = max(0, (q - ix + 1) >> 1);
+    const int ihi = min(nx - 1, (cw4 - 1 + q - ix) >> 1);
+    const int nin = ihi >= ilo ? ihi - ilo + 1 : 0;
+    const int ia = (ilo + 3) & ~3, ib = (ihi + 1) & ~3;  // whole dwords of samples
+    const int hofs = ix >> 1;
+    const uint32_t sh = static_cast<uint32_t>(hofs & 3);
     int acc = 0;
     for (int jj = 0; jj < nj; ++jj) {
         const int r4 = 2 * (j0 + jj) + iy - q;
         if (r4 < 0 || 4 * r4 >= h) { acc += nx * (255 * 4); continue; }
-        const uint8_t *rr = ref_s + (r4 - r4_base) * lc + ix;  // column c4 = 2i + ix - q
-        const uint8_t *cr = cur_s + jj * nx;
-        for (int i = 0; i < nx; ++i) {
-            const int d = abs(static_cast<int>(cr[i]) - static_cast<int>(rr[2 * i]));
-            acc += static_cast<unsigned>(2 * i + ix - q) < cw4 ? d : 255 * 4;
+        acc += (nx - nin) * (255 * 4);
+        const uint8_t *rr = ref_s + (2 * (r4 - r4_base) + (ix & 1)) * hp + hofs;  // rr[i]
+        const uint8_t *cr = cur_s + jj * cp;
+        if (ia >= ib) {
+            for (int i = ilo; i <= ihi; ++i) acc += abs(static_cast<int>(cr[i]) - static_cast<int>(rr[i]));
+            continue;
         }
+        for (int i = ilo; i < ia; ++i) acc += abs(static_cast<int>(cr[i]) - static_cast<int>(rr[i]));
+        const uint32_t *rw = reinterpret_cast<const uint32_t *>(ref_s + (2 * (r4 - r4_base) + (ix & 1)) * hp);
+        const uint32_t *cw = reinterpret_cast<const uint32_t *>(cr);
+        uint32_t vacc = 0;
+        for (int i = ia; i < ib; i += 4) {
+            const int b = (i + hofs) >> 2;
+            vacc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(rw[b + 1], rw[b], sh), cw[i >> 2],
+                                           vacc);
+        }
+        acc += static_cast<int>(vacc);
+        for (int i = ib; i <= ihi; ++i) acc += abs(static_cast<int>(cr[i]) - static_cast<int>(rr[i]));
     }
     part[(static_cast<size_t>(pair) * nbands + band) * kGmvEntries + e] = acc;
 }
@@ -244,13 +274,19 @@ __global__ __launch_bounds__(256) void sad_kernel(const PicDev *__restrict__ tab
 
 }  // namespace
 
+// Staging bytes of a table workgroup: both halves of up to 2(rows-1)+5
+// reference rows, `rows` current sample rows.
+size_t gmv_lds(int rows, int nx)
+{
+    const int nr = 2 * (rows - 1) + 5;
+    return static_cast<size_t>(2 * nr) * gmv_half(nx) + static_cast<size_t>(rows) * gmv_curp(nx);
+}
+
 int gmv_rows(int max_w)
 {
     const int nx = (max_w / 4 + 1) / 2;
-    for (int rows = 16; rows > 1; rows >>= 1) {
-        const int nr = 2 * (rows - 1) + 5, lc = 2 * nx + kGmvSide;
-        if (nr * lc + rows * nx <= kGmvLdsMax) return rows;
-    }
+    for (int rows = 16; rows > 1; rows >>= 1)
+        if (gmv_lds(rows, nx) <= static_cast<size_t>(kGmvLdsMax)) return rows;
     return 1;
 }
 
@@ -267,8 +303,7 @@ hipError_t launch_global_mv(const PicDev *table, int npics, const int32_t *pairs
     if (n <= 0) return hipSuccess;
     const int rows = gmv_rows(max_w), nx = (max_w / 4 + 1) / 2, ny = (max_h / 4 + 1) / 2;
     const int nbands = std::max(1, (ny + rows - 1) / rows);
-    const int nr = 2 * (rows - 1) + 5, lc = 2 * nx + kGmvSide;
-    const size_t lds = static_cast<size_t>(nr) * lc + static_cast<size_t>(rows) * nx;
+    const size_t lds = gmv_lds(rows, nx);
     const int chunks = (kGmvEntries + 255) / 256;
     hipLaunchKernelGGL(gmv_table_kernel, dim3(chunks, nbands, n), dim3(256), lds, s, table, npics,
                        pairs, rows, nbands, part);

@@ -5,7 +5,7 @@
 //                     get_interp_luma's per-sample ladder (src/motion.c:269-304)
 //   mc_chroma_kernel  tc_inter_predict_chroma, src/motion.c:597-632
 //
-// One workgroup per block; a thread produces samples of one row.  Every
+// One wavefront per block, four blocks per workgroup.  Every
 // branch of the reference's luma predictor (row copy for an in-frame integer
 // MV, whole-block six-tap, per-sample six-tap / bilinear / nearest ladder)
 // gives, sample for sample, the ladder's value, so the kernel computes the
@@ -28,29 +28,39 @@ typedef __attribute__((address_space(1))) uint8_t *gout8;
 
 __device__ __forceinline__ int clip255(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
 
-// Lattice entry (Y, X): 16-bit field (Y&1)*2 + (X&1) of anchor (Y/2, X/2),
-// sign-extended (the sample field is 0..255).
-__device__ __forceinline__ int lattice(gptr64 an, int apitch, int Y, int X)
-{
-    const uint64_t a = an[static_cast<size_t>(Y >> 1) * apitch + (X >> 1)];
-    const int sh = ((Y & 1) * 2 + (X & 1)) * 16;
-    return static_cast<int>(static_cast<int16_t>(static_cast<uint16_t>(a >> sh)));
-}
-
 // Bilinear quarter-pel step, rounded per pass (src/motion.c:111-116).
 __device__ __forceinline__ int lerp4(int a, int b, int f) { return (a * (4 - f) + b * f + 2) >> 2; }
 
+// 16-bit field f (0 sample, 1 hh, 2 vv, 3 cc) of an anchor, sign-extended.
+__device__ __forceinline__ int field(uint64_t a, int f)
+{
+    return static_cast<int>(static_cast<int16_t>(static_cast<uint16_t>(a >> (16 * f))));
+}
+
 // get_interp_luma (src/motion.c:269-304) at integer (x,y), fraction (dx,dy).
+// With X0 = (4x+dx) >> 1 and X1 = (4x+dx+1) >> 1 the lattice columns lie in
+// anchor column x, except X1 = 2x+2 (anchor x+1) at dx = 3; rows likewise.
+// So a sample reads its own anchor, and its right / lower neighbours only at
+// phase 3.
 __device__ __forceinline__ int luma_px(const PicDev &R, int x, int y, int dx, int dy)
 {
     const gptr8 Y = (gptr8)R.y;
     if (x >= 2 && x + 3 < R.w && y >= 2 && y + 3 < R.h) {
         if (dx == 0 && dy == 0) return Y[static_cast<size_t>(y) * R.pitch + x];
-        const gptr64 an = (gptr64)R.an;
-        const int qx = 4 * x + dx, qy = 4 * y + dy;
-        const int x0 = qx >> 1, x1 = (qx + 1) >> 1, y0 = qy >> 1, y1 = (qy + 1) >> 1;
-        const int top = (lattice(an, R.apitch, y0, x0) + lattice(an, R.apitch, y0, x1) + 1) >> 1;
-        const int bot = (lattice(an, R.apitch, y1, x0) + lattice(an, R.apitch, y1, x1) + 1) >> 1;
+        const gptr64 an = (gptr64)R.an + static_cast<size_t>(y) * R.apitch + x;
+        const uint64_t a00 = an[0];
+        const uint64_t a10 = dx == 3 ? an[1] : a00;
+        const uint64_t a01 = dy == 3 ? an[R.apitch] : a00;
+        const uint64_t a11 = dx == 3 && dy == 3 ? an[R.apitch + 1] : (dx == 3 ? a10 : a01);
+        // lattice (Y, X) for Y in {Y0, Y1}, X in {X0, X1}: field (Y&1)*2 + (X&1)
+        const int X0 = dx >> 1, X1 = (dx + 1) >> 1, Y0 = dy >> 1, Y1 = (dy + 1) >> 1;
+        const int f00 = (Y0 & 1) * 2 + (X0 & 1), f01 = (Y0 & 1) * 2 + (X1 & 1);
+        const int f10 = (Y1 & 1) * 2 + (X0 & 1), f11 = (Y1 & 1) * 2 + (X1 & 1);
+        const uint64_t r0c1 = X1 == 2 ? a10 : a00;   // row Y0: anchor of column X1
+        const uint64_t r1c0 = Y1 == 2 ? a01 : a00;   // row Y1: anchor of column X0
+        const uint64_t r1c1 = Y1 == 2 ? (X1 == 2 ? a11 : a01) : (X1 == 2 ? a10 : a00);
+        const int top = (field(a00, f00) + field(r0c1, f01) + 1) >> 1;
+        const int bot = (field(r1c0, f10) + field(r1c1, f11) + 1) >> 1;
         return clip255((top + bot + 1) >> 1);
     }
     if (x >= 0 && y >= 0 && x + 1 < R.w && y + 1 < R.h) {
@@ -79,25 +89,37 @@ __device__ __forceinline__ int check_pred(const PicDev *table, int npics, const 
     return 0;
 }
 
+// Lane layout of one wavefront over an n x n block (1 <= n <= 64): lanes
+// take column lane % n of rows lane / n, lane / n + 64 / n, ...; lanes past
+// the last whole row of a pass idle.
+struct BlockLanes {
+    int c, r0, dr;
+    __device__ BlockLanes(int n, int lane) : c(lane % n), r0(lane / n), dr(64 / n)
+    {
+        if (r0 >= dr) r0 = n;  // idle lane
+    }
+};
+
+// One wavefront per predicted block, four blocks per workgroup.
 __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__ table, int npics,
                                                       const tcme_pred *__restrict__ preds, int n,
                                                       uint8_t *__restrict__ out, int64_t out_bytes,
                                                       int32_t *__restrict__ status)
 {
-    const int i = blockIdx.x;
+    const int i = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+    const int lane = static_cast<int>(threadIdx.x & 63);
     if (i >= n) return;
     const tcme_pred p = preds[i];
     PicDev R{};
     const int st = check_pred(table, npics, p, out_bytes, true, R);
-    if (status && threadIdx.x == 0) status[i] = st;
+    if (status && lane == 0) status[i] = st;
     if (st) return;
     const int fx = p.mv_x >> 2, fy = p.mv_y >> 2, dx = p.mv_x & 3, dy = p.mv_y & 3;
     const gout8 o = (gout8)out + p.out_off;
-    for (int k = threadIdx.x; k < p.blk * p.blk; k += blockDim.x) {
-        const int r = k / p.blk, c = k - r * p.blk;
-        o[static_cast<int64_t>(r) * p.out_stride + c] =
-            static_cast<uint8_t>(luma_px(R, fx + c, fy + r, dx, dy));
-    }
+    const BlockLanes bl(p.blk, lane);
+    for (int r = bl.r0; r < p.blk; r += bl.dr)
+        o[static_cast<int64_t>(r) * p.out_stride + bl.c] =
+            static_cast<uint8_t>(luma_px(R, fx + bl.c, fy + r, dx, dy));
 }
 
 // Chroma at 1/8 sample: integer mv >> 3, fraction mv & 7, edge-clamped
@@ -109,20 +131,22 @@ __global__ __launch_bounds__(256) void mc_chroma_kernel(const PicDev *__restrict
                                                         int64_t out_bytes,
                                                         int32_t *__restrict__ status)
 {
-    const int i = blockIdx.x;
+    const int i = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+    const int lane = static_cast<int>(threadIdx.x & 63);
     if (i >= n) return;
     const tcme_pred p = preds[i];
     PicDev R{};
     const int st = check_pred(table, npics, p, out_bytes, false, R);
-    if (status && threadIdx.x == 0) status[i] = st;
+    if (status && lane == 0) status[i] = st;
     if (st) return;
     const int xi = p.mv_x >> 3, yi = p.mv_y >> 3, fx = p.mv_x & 7, fy = p.mv_y & 7;
     const gptr8 Y = (gptr8)R.y;
     const gout8 o = (gout8)out + p.out_off;
-    for (int k = threadIdx.x; k < p.blk * p.blk; k += blockDim.x) {
-        const int r = k / p.blk, c = k - r * p.blk;
+    const BlockLanes bl(p.blk, lane);
+    const int c = bl.c;
+    const int x0 = min(max(xi + c, 0), R.w - 1), x1 = min(max(xi + c + 1, 0), R.w - 1);
+    for (int r = bl.r0; r < p.blk; r += bl.dr) {
         const int y0 = min(max(yi + r, 0), R.h - 1), y1 = min(max(yi + r + 1, 0), R.h - 1);
-        const int x0 = min(max(xi + c, 0), R.w - 1), x1 = min(max(xi + c + 1, 0), R.w - 1);
         const int a = Y[static_cast<size_t>(y0) * R.pitch + x0];
         const int b = Y[static_cast<size_t>(y0) * R.pitch + x1];
         const int cc = Y[static_cast<size_t>(y1) * R.pitch + x0];
@@ -138,11 +162,12 @@ hipError_t launch_predict(bool chroma, const PicDev *table, int npics, const tcm
                           int n, uint8_t *out, int64_t out_bytes, int32_t *status, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
+    const dim3 grid((n + 3) / 4);
     if (chroma)
-        hipLaunchKernelGGL(mc_chroma_kernel, dim3(n), dim3(256), 0, s, table, npics, preds, n, out,
+        hipLaunchKernelGGL(mc_chroma_kernel, grid, dim3(256), 0, s, table, npics, preds, n, out,
                            out_bytes, status);
     else
-        hipLaunchKernelGGL(mc_luma_kernel, dim3(n), dim3(256), 0, s, table, npics, preds, n, out,
+        hipLaunchKernelGGL(mc_luma_kernel, grid, dim3(256), 0, s, table, npics, preds, n, out,
                            out_bytes, status);
     return hipGetLastError();
 }
