@@ -965,7 +965,10 @@ struct GroupWalk {
 // Lanes per call by block size: 64x64 one call per wavefront, 32x32 four,
 // 16x16 eight, 8x8 sixteen (each lane then holds 64, 64, 32 or 16 samples of
 // its block).
-template <int N> struct Lanes { static constexpr int L = N == 64 ? 64 : N == 32 ? 16 : N == 16 ? 8 : 4; };
+template <int N> struct Lanes {
+    static constexpr int L = N == 64 ? 64 : N == 32 ? 16 : N == 16 ? 8 : 4;
+    static constexpr int C = 64 / L;  // calls per slot
+};
 __host__ __device__ constexpr int size_class(int n) { return n == 64 ? 0 : n == 32 ? 1 : n == 16 ? 2 : 3; }
 
 template <int WAVES>
@@ -1250,7 +1253,9 @@ __global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile
         __syncthreads();
         // slots, largest blocks first: 64 (1 call), 32 (4), 16 (8), 8 (16)
         const int n64 = sh.nlist[0], n32 = sh.nlist[1], n16 = sh.nlist[2], n8 = sh.nlist[3];
-        const int s64 = n64, s32 = (n32 + 3) / 4, s16 = (n16 + 7) / 8, s8 = (n8 + 15) / 16;
+        constexpr int C64 = Lanes<64>::C, C32 = Lanes<32>::C, C16 = Lanes<16>::C, C8 = Lanes<8>::C;
+        const int s64 = (n64 + C64 - 1) / C64, s32 = (n32 + C32 - 1) / C32;
+        const int s16 = (n16 + C16 - 1) / C16, s8 = (n8 + C8 - 1) / C8;
         for (;;) {
             int s = fetch_item(&sh.next);
             if constexpr (BIG) {
@@ -1260,18 +1265,18 @@ __global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile
                 }
                 s -= s64;
                 if (s < s32) {
-                    run_slot<32>(sh, table, R, lw, lt, 4 * s, n32 - 4 * s, lane, res, stats);
+                    run_slot<32>(sh, table, R, lw, lt, C32 * s, n32 - C32 * s, lane, res, stats);
                     continue;
                 }
                 s -= s32;
             }
             if (s < s16) {
-                run_slot<16>(sh, table, R, lw, lt, 8 * s, n16 - 8 * s, lane, res, stats);
+                run_slot<16>(sh, table, R, lw, lt, C16 * s, n16 - C16 * s, lane, res, stats);
                 continue;
             }
             s -= s16;
             if (s < s8) {
-                run_slot<8>(sh, table, R, lw, lt, 16 * s, n8 - 16 * s, lane, res, stats);
+                run_slot<8>(sh, table, R, lw, lt, C8 * s, n8 - C8 * s, lane, res, stats);
                 continue;
             }
             break;
