@@ -708,6 +708,21 @@ __device__ __forceinline__ int xcd_slot(int b, int nb)
     return x * q + (x < r ? x : r) + slot;
 }
 
+// Workgroup -> chunk of the tile kernel.  Workgroup b runs on XCD b % 8; in
+// each round of 8K chunks, XCD x takes the K consecutive chunks x*K .. x*K+K-1
+// in turn.  At K = 8 (two CTUs against four references, for CTU-major lists)
+// a CTU's right neighbour against the same reference is searched on the same
+// XCD, whose L2 still holds much of its window, while every XCD works on every
+// part of the frame: a contiguous eighth of the list per XCD left the XCDs
+// with the most motion working alone at the end of the launch.
+__device__ __forceinline__ int tile_slot(int b, int nb)
+{
+    constexpr int K = 8, RR = 8 * K;
+    if (b >= nb / RR * RR) return b;  // the last, partial round in order
+    const int t = b % RR;
+    return b - t + (t % 8) * K + t / 8;
+}
+
 // One workgroup of WPC wavefronts per call, block size N only: each size has
 // its own instance (the register and LDS budgets differ ~16x between 8x8 and
 // 64x64).  A workgroup whose call has another size exits at once; the 64
@@ -1139,7 +1154,7 @@ __global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile
                                                             tcme_stats *__restrict__ stats)
 {
     __shared__ TileShared<WAVES> sh;
-    const int b = xcd_slot(blockIdx.x, gridDim.x);
+    const int b = tile_slot(blockIdx.x, gridDim.x);
     const int c0 = b * chunk;
     if (c0 >= n) return;
     const int cnt = n - c0 < chunk ? n - c0 : chunk;
