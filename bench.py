@@ -49,7 +49,7 @@ def build_workload(eng):
         assert len(dpb) == 4, (poc, dpb)
         sr = workload.SR_B if kind == "B" else workload.SR_P
         frames.append((poc, kind, workload.frame_calls_by_ctu(W, H, [pics[p] for p in dpb],
-                                                              pics[poc], sr)))
+                                                              pics[poc], sr, ref_major=True)))
     return scene, pics, frames
 
 

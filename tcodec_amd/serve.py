@@ -85,9 +85,15 @@ class DeviceGradientClip(Clip):
         self.host = None
 
 
+def frame_template(w: int, h: int, refs: int, sr: int) -> np.ndarray:
+    """A picture's call list with picture ids 0..refs-1 for the references
+    and `refs` for the current picture: every quadtree node against every
+    reference, in 85-call groups, reference-major."""
+    return workload.frame_calls_by_ctu(w, h, list(range(refs)), refs, sr, ref_major=True)
+
+
 class Templates:
-    """Device call-list templates per (w, h, refs, sr): picture ids are slots
-    0..refs-1 for the references and `refs` for the current picture."""
+    """Device call-list templates per (w, h, refs, sr) (``frame_template``)."""
 
     def __init__(self):
         self.cache: dict[tuple, torch.Tensor] = {}
@@ -96,7 +102,7 @@ class Templates:
         key = (w, h, refs, sr)
         t = self.cache.get(key)
         if t is None:
-            calls = workload.frame_calls_by_ctu(w, h, list(range(refs)), refs, sr)
+            calls = frame_template(w, h, refs, sr)
             t = torch.from_numpy(calls.view(np.int32).reshape(-1, 8).copy()).cuda()
             self.cache[key] = t
         return t

@@ -98,11 +98,14 @@ def algo_bytes(stats: np.ndarray, blk: np.ndarray) -> int:
 
 
 def frame_calls_by_ctu(w: int, h: int, ref_pics, cur_pic: int, sr: int,
-                       sizes=SIZES) -> np.ndarray:
+                       sizes=SIZES, ref_major: bool = False) -> np.ndarray:
     """The same call set as ``frame_calls(qt_nodes(w, h), ...)`` ordered for
     the chunked kernel: grouped by (CTU, reference), complete CTUs (85 nodes)
     first and partial edge CTUs last, largest blocks first inside a group, so
-    a chunk of 85 calls is exactly one CTU searched against one reference."""
+    a chunk of 85 calls is exactly one CTU searched against one reference.
+    Groups follow CTU order, a CTU's references in turn; with ``ref_major``
+    every CTU against the first reference comes first, then the second, ...
+    (neighbouring chunks then search neighbouring CTUs in the same picture)."""
     nodes = qt_nodes(w, h, sizes)
     ctu = (nodes[:, 1] // 64) * ((w + 63) // 64) + nodes[:, 0] // 64
     counts = np.bincount(ctu, minlength=int(ctu.max()) + 1)
@@ -111,7 +114,10 @@ def frame_calls_by_ctu(w: int, h: int, ref_pics, cur_pic: int, sr: int,
     m = nodes.shape[0]
     r_idx = np.repeat(np.arange(refs.size), m)
     nd = np.tile(np.arange(m), refs.size)
-    order = np.lexsort((nodes[nd, 0], nodes[nd, 1], -nodes[nd, 2], r_idx, ctu[nd], partial[nd]))
+    if ref_major:
+        order = np.lexsort((nodes[nd, 0], nodes[nd, 1], -nodes[nd, 2], ctu[nd], r_idx, partial[nd]))
+    else:
+        order = np.lexsort((nodes[nd, 0], nodes[nd, 1], -nodes[nd, 2], r_idx, ctu[nd], partial[nd]))
     out = np.zeros(m * refs.size, CALL_DTYPE)
     sel = nd[order]
     out["ref_pic"] = refs[r_idx[order]]

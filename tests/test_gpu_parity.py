@@ -220,9 +220,11 @@ def test_gpu_tilings_agree_with_oracle(engine, orc, tiling):
     old = engine.tiling()
     engine.set_tiling(*tiling)
     try:
-        for order in ("ctu", "ref", "shuffled"):
+        for order in ("ctu", "ref_groups", "ref", "shuffled"):
             if order == "ctu":
                 calls = workload.frame_calls_by_ctu(256, 200, rps, cp, workload.SR_B)
+            elif order == "ref_groups":
+                calls = workload.frame_calls_by_ctu(256, 200, rps, cp, workload.SR_B, ref_major=True)
             elif order == "ref":
                 calls = workload.frame_calls(workload.qt_nodes(256, 200), rps, cp, workload.SR_P)
             else:

@@ -32,8 +32,8 @@ def test_served_results_same_alone_and_in_mix(engine, orc):
     # one stream's last picture against the oracle, call for call
     s = next(x for x in srv.finished if x.sid == 0)
     poc, kind, dpb = s.order[-1]
-    calls = workload.frame_calls_by_ctu(s.w, s.h, list(range(len(dpb))), len(dpb),
-                                        workload.SR_B if kind == "B" else workload.SR_P)
+    calls = serve.frame_template(s.w, s.h, len(dpb),
+                                 workload.SR_B if kind == "B" else workload.SR_P)
     plane = lambda p: s.clip.host[(s.offset + p) % s.clip.frames]
     got = s.results[-1]
     for k, p in enumerate(dpb):
