@@ -181,6 +181,9 @@ class Reference:
         L.refx_qt_mvp.argtypes = [C.c_int, C.c_int, i16p, i16p, u8p, i32p, i32p]
         L.refx_global_mv.restype = C.c_int
         L.refx_global_mv.argtypes = [u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int, i32p, i32p]
+        L.refx_motion_est_batch.restype = C.c_int
+        L.refx_motion_est_batch.argtypes = [u8p, C.c_int, C.c_int, C.c_int, u8p, C.c_int, i32p,
+                                            C.c_int, i32p]
 
     def motion_est(self, ref, cur, px, py, cx, cy, n, sr, w=None, h=None):
         hh, ww = ref.shape
@@ -191,6 +194,17 @@ class Reference:
                                     C.cast(cur.ctypes.data + int(py) * cur.strides[0] + int(px), u8p),
                                     cur.strides[0], cx, cy, n, sr, C.byref(sad))
         return mv.x, mv.y, sad.value
+
+    def motion_est_batch(self, ref, cur, calls):
+        """calls: int32 [N,6] = px, py, cx, cy, n, sr -> int32 [N,3], the
+        reference's tc_motion_est for each (one foreign call: the GIL is
+        released for the whole batch)."""
+        calls = np.ascontiguousarray(calls, dtype=np.int32)
+        out = np.zeros((calls.shape[0], 3), np.int32)
+        h, w = ref.shape
+        self.lib.refx_motion_est_batch(ptr(ref), ref.strides[0], w, h, ptr(cur), cur.strides[0],
+                                       ptr(calls, i32p), calls.shape[0], ptr(out, i32p))
+        return out
 
     def inter_predict(self, ref, mvx, mvy, n, w=None, h=None):
         hh, ww = ref.shape

@@ -9,6 +9,11 @@
  *   refx_qt_mvp        -> qt_mvp              (src/encoder.c:405-427)
  *   refx_global_mv     -> estimate_global_mv  (src/encoder.c:171-202)
  *
+ * and one loop over the reference's public tc_motion_est (src/motion.c:384),
+ * so that a CPU baseline can hand a thread many calls in one foreign call:
+ *
+ *   refx_motion_est_batch
+ *
  * Built only into oracle/_ref/libtcref.so by oracle/Makefile.
  */
 #include "encoder.c"
@@ -46,5 +51,24 @@ int refx_global_mv(const uint8_t *cur, int cur_stride,
     *gx = enc->glob_mv_x;
     *gy = enc->glob_mv_y;
     free(enc);
+    return 0;
+}
+
+/* calls: n rows of (px, py, cx, cy, blk, sr); out: n rows of (mv_x, mv_y,
+ * sad), each exactly tc_motion_est's result for the block at (px, py) of cur. */
+int refx_motion_est_batch(const uint8_t *ref, int ref_stride, int w, int h,
+                          const uint8_t *cur, int cur_stride, const int32_t *calls, int n,
+                          int32_t *out)
+{
+    for (int i = 0; i < n; i++) {
+        const int32_t *c = calls + 6 * i;
+        tc_sad_t sad = 0;
+        tc_mv_s mv = tc_motion_est(ref, ref_stride, w, h,
+                                   cur + (size_t)c[1] * cur_stride + c[0], cur_stride,
+                                   c[2], c[3], c[4], c[5], &sad);
+        out[3 * i] = mv.x;
+        out[3 * i + 1] = mv.y;
+        out[3 * i + 2] = sad;
+    }
     return 0;
 }
