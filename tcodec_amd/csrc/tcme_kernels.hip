@@ -3,28 +3,32 @@
 //   anchor_kernel     per reference picture: every sample with its raw six-tap
 //                     half samples, packed 8 bytes per sample (reference
 //                     src/motion.c:70-107)
-//   me_search_kernel  the hierarchical hexagon search + quarter-pel refinement
-//                     (reference src/motion.c:384-485): one workgroup per call,
-//                     WPC wavefronts splitting its block, one instance per
-//                     block size
-//   me_tile_kernel    the same search for the 8x8 and 16x16 calls, one
-//                     wavefront per call, many calls per workgroup sharing a
-//                     reference window staged in LDS
+//   me_tile_kernel    the hierarchical hexagon search + quarter-pel refinement
+//                     (reference src/motion.c:384-485) for many calls per
+//                     workgroup: runs of calls that share a reference picture
+//                     and a 64x64 centre tile search from a luma window and a
+//                     half-sample lattice staged in LDS once, each wavefront
+//                     walking several calls side by side (L lanes per call).
+//                     The <8, true> instance takes every block size and is
+//                     the default: one launch per frame.
+//   me_search_kernel  the same search, one workgroup per call, WPC wavefronts
+//                     splitting its block, one instance per block size: the
+//                     drop-in's single calls and the other launch shapes.
 //
-// The walk is the reference's sequential greedy walk.  Its decisions are
-// wave-uniform (scalar unit); lanes price candidates.  Candidates may be
-// priced speculatively in batches around the current best: the FIRST one in
+// The walk is the reference's sequential greedy walk.  In the tile kernel
+// each group of L lanes runs it for its own call on vector registers.  In the
+// per-call kernel its decisions are wave-uniform (scalar unit) and lanes price
+// candidates, possibly in batches around the current best: the FIRST one in
 // table order that is strictly better is taken and those after it are priced
 // again around the new best.  That is exactly what the sequential loop
 // computes, because the candidates before the first improvement saw the same
 // centre, so results are bit-identical whatever the batch size.
 //
-// Integer block costs: v_sad_u8 on packed bytes (4 |a-b| per lane-op), a DPP
-// row scan per wavefront and, when several wavefronts share a block, an LDS
-// exchange.  Quarter-pel: every candidate of one call lies within one sample
-// of the integer best (fx in {X-1, X}, fy in {Y-1, Y}), so the call stages the
-// (N+2) x (N+2) anchor window once in LDS and prices each candidate with code
-// specialised for its phase.
+// Integer block costs: v_sad_u8 on packed bytes (4 |a-b| per lane-op) and DPP
+// sums (an LDS exchange when several wavefronts share a block).  Quarter-pel:
+// every candidate of one call lies within one sample of the integer best, so
+// the six-tap values come from the run's lattice or from an (N+2) x (N+2)
+// anchor window staged once in LDS.
 #include "tcme_internal.h"
 
 namespace tcme {

@@ -288,7 +288,7 @@ class Engine:
         return t
 
     def set_tiling(self, waves8: int, waves16: int, waves32: int, waves64: int,
-                   tile_waves: int = 8, tile_chunk: int = 85, tile_big: int = 0) -> None:
+                   tile_waves: int = 8, tile_chunk: int = 85, tile_big: int = 1) -> None:
         t = Tiling(waves8, waves16, waves32, waves64, tile_waves, tile_chunk, tile_big)
         _check(self.L.tcme_set_tiling(self.h, C.byref(t)))
 
