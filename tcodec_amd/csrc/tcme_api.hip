@@ -218,6 +218,25 @@ int pic_derive(tcme_ctx *c, int id, hipStream_t s)
 
 }  // namespace
 
+namespace tcme {
+
+int device_cus()
+{
+    static int cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cus[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            v <= 0)
+            v = 256;
+        cus[dev] = v;
+    }
+    return cus[dev];
+}
+
+}  // namespace tcme
+
 TCME_API const char *tcme_last_error(void) { return g_err.c_str(); }
 
 TCME_API const char *tcme_version(void) { return "tcme 0.1 gfx950"; }

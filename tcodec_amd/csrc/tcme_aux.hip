@@ -221,55 +221,58 @@ __global__ __launch_bounds__(kGmvThreads) void gmv_walk_kernel(const PicDev *__r
     }
 }
 
-// One wavefront per descriptor; lanes take samples, a DPP scan sums them.
+// One wavefront per descriptor, the wavefronts striding over the list; lanes
+// take samples, a DPP scan sums them.
 __global__ __launch_bounds__(256) void sad_kernel(const PicDev *__restrict__ table, int npics,
                                                   const tcme_sad_desc *__restrict__ descs, int n,
                                                   const uint8_t *__restrict__ buf,
                                                   int64_t buf_bytes, int32_t *__restrict__ out)
 {
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (i >= n) return;  // a whole wavefront leaves together
-    const tcme_sad_desc d = descs[i];
-    int status = 0;
-    PicDev A{}, B{};
-    if (d.pic < 0 || d.pic >= npics || d.other_pic >= npics) status = TCME_E_PIC;
-    else if (d.w < 1 || d.h < 1 || d.w > kMaxBlk || d.h > kMaxBlk) status = TCME_E_BLK;
-    else {
-        A = table[d.pic];
-        if (!A.y) status = TCME_E_PIC;
-        else if (d.x < 0 || d.y < 0 || d.x + d.w > A.w || d.y + d.h > A.h) status = TCME_E_ARG;
-        else if (d.other_pic >= 0) {
-            B = table[d.other_pic];
-            if (!B.y) status = TCME_E_PIC;
-            else if (d.ox < 0 || d.oy < 0 || d.ox + d.w > B.w || d.oy + d.h > B.h)
+    const int stride = static_cast<int>(gridDim.x) * 4;
+    for (int i = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6); i < n;
+         i += stride) {
+        const tcme_sad_desc d = descs[i];
+        int status = 0;
+        PicDev A{}, B{};
+        if (d.pic < 0 || d.pic >= npics || d.other_pic >= npics) status = TCME_E_PIC;
+        else if (d.w < 1 || d.h < 1 || d.w > kMaxBlk || d.h > kMaxBlk) status = TCME_E_BLK;
+        else {
+            A = table[d.pic];
+            if (!A.y) status = TCME_E_PIC;
+            else if (d.x < 0 || d.y < 0 || d.x + d.w > A.w || d.y + d.h > A.h) status = TCME_E_ARG;
+            else if (d.other_pic >= 0) {
+                B = table[d.other_pic];
+                if (!B.y) status = TCME_E_PIC;
+                else if (d.ox < 0 || d.oy < 0 || d.ox + d.w > B.w || d.oy + d.h > B.h)
+                    status = TCME_E_ARG;
+            } else if (!buf || d.buf_stride < d.w || d.buf_off < 0 ||
+                       d.buf_off + static_cast<int64_t>(d.h - 1) * d.buf_stride + d.w > buf_bytes) {
                 status = TCME_E_ARG;
-        } else if (!buf || d.buf_stride < d.w || d.buf_off < 0 ||
-                   d.buf_off + static_cast<int64_t>(d.h - 1) * d.buf_stride + d.w > buf_bytes) {
-            status = TCME_E_ARG;
+            }
         }
+        if (status) {
+            if (lane == 0) out[i] = status;
+            continue;
+        }
+        const gptr8 a = (gptr8)A.y + static_cast<size_t>(d.y) * A.pitch + d.x;
+        const gptr8 b = d.other_pic >= 0 ? (gptr8)B.y + static_cast<size_t>(d.oy) * B.pitch + d.ox
+                                         : (gptr8)buf + d.buf_off;
+        const int bp = d.other_pic >= 0 ? B.pitch : d.buf_stride;
+        int s = 0;
+        for (int k = lane; k < d.w * d.h; k += 64) {
+            const int r = k / d.w, c = k - r * d.w;
+            const int p = a[static_cast<size_t>(r) * A.pitch + c], q = b[static_cast<size_t>(r) * bp + c];
+            s += p > q ? p - q : q - p;
+        }
+        s += __builtin_amdgcn_update_dpp(0, s, 0x111, 0xF, 0xF, false);
+        s += __builtin_amdgcn_update_dpp(0, s, 0x112, 0xF, 0xF, false);
+        s += __builtin_amdgcn_update_dpp(0, s, 0x114, 0xF, 0xF, false);
+        s += __builtin_amdgcn_update_dpp(0, s, 0x118, 0xF, 0xF, false);
+        const int t = __builtin_amdgcn_readlane(s, 15) + __builtin_amdgcn_readlane(s, 31) +
+                      __builtin_amdgcn_readlane(s, 47) + __builtin_amdgcn_readlane(s, 63);
+        if (lane == 0) out[i] = t;
     }
-    if (status) {
-        if (lane == 0) out[i] = status;
-        return;
-    }
-    const gptr8 a = (gptr8)A.y + static_cast<size_t>(d.y) * A.pitch + d.x;
-    const gptr8 b = d.other_pic >= 0 ? (gptr8)B.y + static_cast<size_t>(d.oy) * B.pitch + d.ox
-                                     : (gptr8)buf + d.buf_off;
-    const int bp = d.other_pic >= 0 ? B.pitch : d.buf_stride;
-    int s = 0;
-    for (int k = lane; k < d.w * d.h; k += 64) {
-        const int r = k / d.w, c = k - r * d.w;
-        const int p = a[static_cast<size_t>(r) * A.pitch + c], q = b[static_cast<size_t>(r) * bp + c];
-        s += p > q ? p - q : q - p;
-    }
-    s += __builtin_amdgcn_update_dpp(0, s, 0x111, 0xF, 0xF, false);
-    s += __builtin_amdgcn_update_dpp(0, s, 0x112, 0xF, 0xF, false);
-    s += __builtin_amdgcn_update_dpp(0, s, 0x114, 0xF, 0xF, false);
-    s += __builtin_amdgcn_update_dpp(0, s, 0x118, 0xF, 0xF, false);
-    const int t = __builtin_amdgcn_readlane(s, 15) + __builtin_amdgcn_readlane(s, 31) +
-                  __builtin_amdgcn_readlane(s, 47) + __builtin_amdgcn_readlane(s, 63);
-    if (lane == 0) out[i] = t;
 }
 
 }  // namespace
@@ -317,8 +320,8 @@ hipError_t launch_sad(const PicDev *table, int npics, const tcme_sad_desc *descs
                       const uint8_t *buf, int64_t buf_bytes, int32_t *out, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sad_kernel, dim3((n + 3) / 4), dim3(256), 0, s, table, npics, descs, n, buf,
-                       buf_bytes, out);
+    hipLaunchKernelGGL(sad_kernel, dim3(std::min((n + 3) / 4, 8 * device_cus())), dim3(256), 0, s,
+                       table, npics, descs, n, buf, buf_bytes, out);
     return hipGetLastError();
 }
 

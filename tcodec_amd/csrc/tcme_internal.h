@@ -38,6 +38,8 @@ struct Tiling {
 };
 
 bool valid_waves(int blk, int waves);
+// Compute units of the current device (cached per device; 256 if unknown).
+int device_cus();
 // Global-motion estimates and block SADs (tcme_aux.hip).  The estimate
 // needs gmv_part_bytes(n, W, H) bytes of workspace, W x H bounding every
 // picture a pair may name.

@@ -5,7 +5,7 @@
 //                     get_interp_luma's per-sample ladder (src/motion.c:269-304)
 //   mc_chroma_kernel  tc_inter_predict_chroma, src/motion.c:597-632
 //
-// One wavefront per block, four blocks per workgroup.  Every
+// One wavefront per block, the wavefronts striding over the list.  Every
 // branch of the reference's luma predictor (row copy for an in-frame integer
 // MV, whole-block six-tap, per-sample six-tap / bilinear / nearest ladder)
 // gives, sample for sample, the ladder's value, so the kernel computes the
@@ -17,6 +17,8 @@
 //   v = clip((avg(M[Y0][X0], M[Y0][X1]) + avg(M[Y1][X0], M[Y1][X1]) + 1) >> 1),
 // avg(a,b) = (a+b+1) >> 1 on raw values (src/motion.c:193-260; the h- and
 // v-only paths of src/motion.c:133-191 agree after their final clip).
+#include <algorithm>
+
 #include "tcme_internal.h"
 
 namespace tcme {
@@ -100,26 +102,31 @@ struct BlockLanes {
     }
 };
 
-// One wavefront per predicted block, four blocks per workgroup.
+// One wavefront per predicted block; the grid is at most a few workgroups
+// per CU and each wavefront strides over the list (a block is only ~2,000
+// cycles of work, so one workgroup per four blocks left the dispatcher, not
+// the CUs, setting the pace).
 __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__ table, int npics,
                                                       const tcme_pred *__restrict__ preds, int n,
                                                       uint8_t *__restrict__ out, int64_t out_bytes,
                                                       int32_t *__restrict__ status)
 {
-    const int i = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
     const int lane = static_cast<int>(threadIdx.x & 63);
-    if (i >= n) return;
-    const tcme_pred p = preds[i];
-    PicDev R{};
-    const int st = check_pred(table, npics, p, out_bytes, true, R);
-    if (status && lane == 0) status[i] = st;
-    if (st) return;
-    const int fx = p.mv_x >> 2, fy = p.mv_y >> 2, dx = p.mv_x & 3, dy = p.mv_y & 3;
-    const gout8 o = (gout8)out + p.out_off;
-    const BlockLanes bl(p.blk, lane);
-    for (int r = bl.r0; r < p.blk; r += bl.dr)
-        o[static_cast<int64_t>(r) * p.out_stride + bl.c] =
-            static_cast<uint8_t>(luma_px(R, fx + bl.c, fy + r, dx, dy));
+    const int stride = static_cast<int>(gridDim.x) * 4;
+    for (int i = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6); i < n;
+         i += stride) {
+        const tcme_pred p = preds[i];
+        PicDev R{};
+        const int st = check_pred(table, npics, p, out_bytes, true, R);
+        if (status && lane == 0) status[i] = st;
+        if (st) continue;
+        const int fx = p.mv_x >> 2, fy = p.mv_y >> 2, dx = p.mv_x & 3, dy = p.mv_y & 3;
+        const gout8 o = (gout8)out + p.out_off;
+        const BlockLanes bl(p.blk, lane);
+        for (int r = bl.r0; r < p.blk; r += bl.dr)
+            o[static_cast<int64_t>(r) * p.out_stride + bl.c] =
+                static_cast<uint8_t>(luma_px(R, fx + bl.c, fy + r, dx, dy));
+    }
 }
 
 // Chroma at 1/8 sample: integer mv >> 3, fraction mv & 7, edge-clamped
@@ -131,28 +138,30 @@ __global__ __launch_bounds__(256) void mc_chroma_kernel(const PicDev *__restrict
                                                         int64_t out_bytes,
                                                         int32_t *__restrict__ status)
 {
-    const int i = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
     const int lane = static_cast<int>(threadIdx.x & 63);
-    if (i >= n) return;
-    const tcme_pred p = preds[i];
-    PicDev R{};
-    const int st = check_pred(table, npics, p, out_bytes, false, R);
-    if (status && lane == 0) status[i] = st;
-    if (st) return;
-    const int xi = p.mv_x >> 3, yi = p.mv_y >> 3, fx = p.mv_x & 7, fy = p.mv_y & 7;
-    const gptr8 Y = (gptr8)R.y;
-    const gout8 o = (gout8)out + p.out_off;
-    const BlockLanes bl(p.blk, lane);
-    const int c = bl.c;
-    const int x0 = min(max(xi + c, 0), R.w - 1), x1 = min(max(xi + c + 1, 0), R.w - 1);
-    for (int r = bl.r0; r < p.blk; r += bl.dr) {
-        const int y0 = min(max(yi + r, 0), R.h - 1), y1 = min(max(yi + r + 1, 0), R.h - 1);
-        const int a = Y[static_cast<size_t>(y0) * R.pitch + x0];
-        const int b = Y[static_cast<size_t>(y0) * R.pitch + x1];
-        const int cc = Y[static_cast<size_t>(y1) * R.pitch + x0];
-        const int d = Y[static_cast<size_t>(y1) * R.pitch + x1];
-        const int v = (8 - fx) * (8 - fy) * a + fx * (8 - fy) * b + (8 - fx) * fy * cc + fx * fy * d;
-        o[static_cast<int64_t>(r) * p.out_stride + c] = static_cast<uint8_t>((v + 32) >> 6);
+    const int stride = static_cast<int>(gridDim.x) * 4;
+    for (int i = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6); i < n;
+         i += stride) {
+        const tcme_pred p = preds[i];
+        PicDev R{};
+        const int st = check_pred(table, npics, p, out_bytes, false, R);
+        if (status && lane == 0) status[i] = st;
+        if (st) continue;
+        const int xi = p.mv_x >> 3, yi = p.mv_y >> 3, fx = p.mv_x & 7, fy = p.mv_y & 7;
+        const gptr8 Y = (gptr8)R.y;
+        const gout8 o = (gout8)out + p.out_off;
+        const BlockLanes bl(p.blk, lane);
+        const int c = bl.c;
+        const int x0 = min(max(xi + c, 0), R.w - 1), x1 = min(max(xi + c + 1, 0), R.w - 1);
+        for (int r = bl.r0; r < p.blk; r += bl.dr) {
+            const int y0 = min(max(yi + r, 0), R.h - 1), y1 = min(max(yi + r + 1, 0), R.h - 1);
+            const int a = Y[static_cast<size_t>(y0) * R.pitch + x0];
+            const int b = Y[static_cast<size_t>(y0) * R.pitch + x1];
+            const int cc = Y[static_cast<size_t>(y1) * R.pitch + x0];
+            const int d = Y[static_cast<size_t>(y1) * R.pitch + x1];
+            const int v = (8 - fx) * (8 - fy) * a + fx * (8 - fy) * b + (8 - fx) * fy * cc + fx * fy * d;
+            o[static_cast<int64_t>(r) * p.out_stride + c] = static_cast<uint8_t>((v + 32) >> 6);
+        }
     }
 }
 
@@ -162,7 +171,7 @@ hipError_t launch_predict(bool chroma, const PicDev *table, int npics, const tcm
                           int n, uint8_t *out, int64_t out_bytes, int32_t *status, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    const dim3 grid((n + 3) / 4);
+    const dim3 grid(std::min((n + 3) / 4, 8 * device_cus()));
     if (chroma)
         hipLaunchKernelGGL(mc_chroma_kernel, grid, dim3(256), 0, s, table, npics, preds, n, out,
                            out_bytes, status);
