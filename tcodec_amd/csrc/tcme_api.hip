@@ -426,9 +426,10 @@ TCME_API int tcme_get_tiling(tcme_ctx *c, tcme_tiling *t)
     return TCME_OK;
 }
 
-// Times every launch shape (wavefronts per 32x32 and per 64x64 call, tile
-// workgroups of 0, 4 or 8 wavefronts for the 8x8/16x16 calls or of 4 for
-// every call; the tile chunk stays the context's) on
+// Times every distinct launch shape (tile workgroups of 4 or 8 wavefronts for
+// every call; or tile workgroups of 0, 4 or 8 wavefronts for the 8x8/16x16
+// calls with each count of wavefronts per 32x32 and per 64x64 call; the tile
+// chunk stays the context's) on
 // the caller's call list (device arrays; results are overwritten with the
 // same answers) and keeps the fastest in the context.  Results do not depend
 // on the shape, only time does.
@@ -440,37 +441,50 @@ TCME_API int tcme_autotune(tcme_ctx *c, const tcme_call *calls, int n, tcme_resu
     if (reps < 1) reps = 3;
     if (set_device(c) != TCME_OK) return TCME_E_HIP;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    static const int w32[] = {1, 2, 4};
-    static const int w64[] = {2, 4, 8};
-    static const int tw[] = {0, 4, 8, -4, -8};  // negative: tile_big with that many wavefronts
-    const int chunk = c->tiling.tile_chunk;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    tcme::Tiling best;
-    float best_t = 1e30f;
-    for (int a : w32)
-        for (int b : w64)
-            for (int tl : tw) {
+    // the distinct shapes: tile_big workgroups of 4 or 8 wavefronts (the
+    // per-call counts unused), then tile workgroups of 0, 4 or 8 wavefronts
+    // for the small calls with every per-call count for 32x32 and 64x64
+    std::vector<tcme::Tiling> shapes;
+    for (int tw : {8, 4}) {
+        tcme::Tiling t;
+        t.tile_waves = tw;
+        t.tile_big = 1;
+        shapes.push_back(t);
+    }
+    for (int tw : {0, 4, 8})
+        for (int a : {1, 2, 4})
+            for (int b : {2, 4, 8}) {
                 tcme::Tiling t;
                 t.waves[2] = a;
                 t.waves[3] = b;
-                t.tile_waves = tl < 0 ? -tl : tl;
-                t.tile_big = tl < 0;
-                t.tile_chunk = chunk;
-                if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;  // warm
-                HIP_TRY(hipEventRecord(e0, s));
-                for (int r = 0; r < reps; ++r)
-                    if (int e = dispatch(c, calls, n, t, results, nullptr, s)) return e;
-                HIP_TRY(hipEventRecord(e1, s));
-                HIP_TRY(hipEventSynchronize(e1));
-                float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                ms /= reps;
-                if (ms < best_t) { best_t = ms; best = t; }
+                t.tile_waves = tw;
+                t.tile_big = 0;
+                shapes.push_back(t);
             }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    for (tcme::Tiling &t : shapes) t.tile_chunk = c->tiling.tile_chunk;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int err = TCME_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        err = fail(TCME_E_HIP, "timing events");
+    tcme::Tiling best = c->tiling;
+    float best_t = 1e30f;
+    for (size_t k = 0; k < shapes.size() && !err; ++k) {
+        const tcme::Tiling &t = shapes[k];
+        err = dispatch(c, calls, n, t, results, nullptr, s);  // warm
+        if (!err && hipEventRecord(e0, s) != hipSuccess) err = fail(TCME_E_HIP, "event record");
+        for (int r = 0; r < reps && !err; ++r) err = dispatch(c, calls, n, t, results, nullptr, s);
+        float ms = 0;
+        if (!err && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            err = fail(TCME_E_HIP, "event timing");
+        if (!err && ms / reps < best_t) {
+            best_t = ms / reps;
+            best = t;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (err) return err;
     c->tiling = best;
     if (best_out) tcme_get_tiling(c, best_out);
     if (best_ms) *best_ms = best_t;
