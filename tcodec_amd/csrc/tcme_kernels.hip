@@ -44,6 +44,8 @@ __device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f)
 // Global-memory views (address space 1), so plane reads compile to global
 // loads rather than flat ones.
 typedef const __attribute__((address_space(1))) uint8_t *gptr8;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) u32x4 *gptr128;
 typedef const __attribute__((address_space(1))) uint32_t *gptr32;
 typedef const __attribute__((address_space(1))) uint64_t *gptr64;
 
@@ -1219,13 +1221,23 @@ __global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile
             // rows start 8-byte aligned (x0 is a multiple of 8, pitch of 64);
             // the last dword may run past x1 into the row's padding, the next
             // row or the plane's tail slack
-            const int dw = (lw.x1 - lw.x0 + 3) >> 2;
+            // 16-byte reads: a row's last one may run up to 15 bytes past x1
+            // (row padding, the next row or the plane's tail slack); the LDS
+            // row holds kWinPitch / 4 >= 4 * q4 dwords
+            const int q4 = (((lw.x1 - lw.x0 + 3) >> 2) + 3) >> 2;
             const int rows = lw.y1 - lw.y0;
-            const gptr32 plane = as_global32(R.y);
-            for (int i = static_cast<int>(threadIdx.x); i < rows * dw; i += 64 * WAVES) {
-                const int r = i / dw, d = i - r * dw;
-                sh.luma[r * (kWinPitch / 4) + d] =
-                    plane[(static_cast<size_t>(lw.y0 + r) * R.pitch + lw.x0) / 4 + d];
+            const gptr8 plane = (gptr8)R.y;
+            static_assert(4 * ((kWinSide / 4 + 3) / 4) <= kWinPitch / 4, "row of 16-byte reads");
+#pragma unroll 4
+            for (int i = static_cast<int>(threadIdx.x); i < rows * q4; i += 64 * WAVES) {
+                const int r = i / q4, k = i - r * q4;
+                const size_t g = static_cast<size_t>(lw.y0 + r) * R.pitch + lw.x0 + 16 * k;
+                const u32x4 v = *(gptr128)(plane + g);
+                uint32_t *dst = sh.luma + r * (kWinPitch / 4) + 4 * k;
+                dst[0] = v.x;
+                dst[1] = v.y;
+                dst[2] = v.z;
+                dst[3] = v.w;
             }
         } else {
             lw.x1 = lw.x0;  // nothing staged: every candidate reads the plane
@@ -1237,13 +1249,24 @@ __global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile
         lt.x1 = tx * 64 + 64 + kLatMargin > R.w ? R.w : tx * 64 + 64 + kLatMargin;
         lt.y1 = ty * 64 + 64 + kLatMargin > R.h ? R.h : ty * 64 + 64 + kLatMargin;
         if (lt.x1 > lt.x0 && lt.y1 > lt.y0) {
+            // two anchors per 16-byte read (x0 and the anchor pitch are even);
+            // with an odd width the last read takes one anchor past x1 (the
+            // next row or the plane's tail slack) into lattice column x1 - x0
+            // < kLatPitch
             const int cols = lt.x1 - lt.x0, rows = lt.y1 - lt.y0;
-            const gptr64 an = as_global64(R.an);
-            for (int i = static_cast<int>(threadIdx.x); i < rows * cols; i += 64 * WAVES) {
-                const int r = i / cols, c = i - r * cols;
-                const uint64_t a = an[static_cast<size_t>(lt.y0 + r) * R.apitch + lt.x0 + c];
-                sh.lat[(2 * r) * kLatPitch + c] = static_cast<uint32_t>(a);
-                sh.lat[(2 * r + 1) * kLatPitch + c] = static_cast<uint32_t>(a >> 32);
+            const int c2 = (cols + 1) >> 1;
+            const gptr128 an2 = (gptr128)R.an;
+            static_assert(kLatSide + 1 <= kLatPitch, "odd width's extra column");
+#pragma unroll 4
+            for (int i = static_cast<int>(threadIdx.x); i < rows * c2; i += 64 * WAVES) {
+                const int r = i / c2, c = 2 * (i - r * c2);
+                const u32x4 v = an2[(static_cast<size_t>(lt.y0 + r) * R.apitch + lt.x0 + c) >> 1];
+                uint32_t *lo = sh.lat + (2 * r) * kLatPitch + c;
+                uint32_t *hi = lo + kLatPitch;
+                lo[0] = v.x;
+                hi[0] = v.y;
+                lo[1] = v.z;
+                hi[1] = v.w;
             }
         } else {
             lt.x1 = lt.x0;  // empty: every call is deferred
