@@ -16,6 +16,8 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
+import subprocess
 import sys
 import time
 
@@ -247,6 +249,43 @@ def dropin_rate(frames, scene, pics, samples: int = 400):
         call(calls[i])
     dt = time.perf_counter() - t0
     return samples / dt / len(calls)
+
+
+def encoder_leg():
+    """The reference encoder (oracle/_ref/tcenc, built from its sources) on the
+    CPU-runnable configuration - 320x240, 8 frames of the diagonal-gradient
+    source (+3,-2) px per frame with noise, v2, P-only, one reference, QP 32 -
+    and the same encoder linked to the drop-in (oracle/_ref/tcenc_gpu): wall
+    time of each process (the GPU one includes its HIP start-up) and whether
+    the .tcv bytes are identical."""
+    import tempfile
+
+    from tcodec_amd import synth
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    exes = {"cpu": os.path.join(ref, "tcenc"), "gpu_dropin": os.path.join(ref, "tcenc_gpu")}
+    if not all(os.path.exists(e) for e in exes.values()):
+        return None
+    out = {"clip": "320x240, 8 frames, diagonal gradient + noise, v2, P-only, 1 reference, QP 32"}
+    with tempfile.TemporaryDirectory() as td:
+        yuv = os.path.join(td, "in.yuv")
+        synth.write_yuv(yuv, synth.gradient_video(320, 240, 8, seed=1))
+        tcv = {}
+        for name, exe in exes.items():
+            tcv[name] = os.path.join(td, name + ".tcv")
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "-w", "320", "-h", "240", "--v2", "-q", "32", "-o", tcv[name], yuv],
+                               check=True, capture_output=True, text=True, timeout=300)
+            dt = time.perf_counter() - t0
+            # the encoder's own clock (its "Time: N sec" summary line) leaves out
+            # process and HIP start-up; the wall time includes them
+            m = re.search(r"Time:\s*([0-9.]+)\s*sec", r.stdout + r.stderr)
+            enc = float(m.group(1)) if m else None
+            out[name + "_wall_s"] = round(dt, 3)
+            out[name + "_encode_s"] = enc
+            out[name + "_frames_per_s"] = round(8 / enc, 2) if enc else None
+        out["tcv_identical"] = open(tcv["cpu"], "rb").read() == open(tcv["gpu_dropin"], "rb").read()
+    out["cores"] = 1
+    return out
 
 
 def mc_leg(eng, frames, d_res, reps: int = 5):
@@ -501,6 +540,11 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(scene, pics, frames, d_res[0].cpu().numpy())
     mc = mc_leg(eng, frames, d_res)
     aux = aux_leg(eng, frames, pics)
+    try:
+        encoder = None if args.no_cpu else encoder_leg()
+    except (OSError, subprocess.SubprocessError) as e:
+        encoder = {"error": f"{type(e).__name__}: {e}"[:300]}
+    log(f"encoder: {encoder}")
     dropin = None if args.no_forms else dropin_rate(frames, scene, pics)
     batched = None if args.no_forms else batched_form(eng, args.batch_steps)
     served = None if args.no_forms else served_form(eng, args.serve_rate, args.serve_seconds,
@@ -551,6 +595,7 @@ def main():
                             if dropin else None),
         "mc": mc,
         "aux": aux,
+        "encoder": encoder,
         "batched": batched,
         "served": served,
     }
