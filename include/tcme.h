@@ -90,7 +90,9 @@ int  tcme_pic_create(tcme_ctx *ctx, int w, int h);
 int  tcme_pic_create_plane(tcme_ctx *ctx, int w, int h);
 int  tcme_pic_destroy(tcme_ctx *ctx, int pic);
 /* Fills a picture from host (upload) or device memory, then rebuilds its
- * anchor plane (if it has one).  Asynchronous on stream (a hipStream_t). */
+ * anchor plane (if it has one).  Asynchronous on stream (a hipStream_t); an
+ * upload has copied the host plane into pinned staging by the time it returns,
+ * so the caller may reuse that buffer at once. */
 int  tcme_pic_upload(tcme_ctx *ctx, int pic, const uint8_t *host, int stride, void *stream);
 int  tcme_pic_upload_device(tcme_ctx *ctx, int pic, const uint8_t *dev, int stride,
                             void *stream);

@@ -1,15 +1,17 @@
 // Host runtime and C ABI of libtcme.so (declared in include/tcme.h).
 //
 // Batch API: a context owns a device picture table; pictures are luma planes
-// with their anchor planes; tcme_search() launches the four per-size search
-// kernels in the context's launch shape.
+// with their anchor planes; tcme_search() launches the search kernels in the
+// context's launch shape (by default one tile-kernel launch for every call).
 //
 // Drop-in: tc_motion_est() keeps device mirrors of the reference planes it is
 // handed.  Before each call it compares, byte for byte, the part of the host
 // plane that call can read (the reachable search window plus the six-tap
 // apron) with the mirror's host shadow and re-uploads on any difference, so a
 // result is always computed from exactly the bytes the caller passed, however
-// the caller reuses its buffers.  The current block travels with the call.
+// the caller reuses its buffers.  The call itself is a kernel argument; the
+// kernel reads the current block from, and writes the result to, pinned host
+// memory, so a call is one launch and one synchronisation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -75,6 +77,13 @@ struct tcme_ctx {
     size_t gmv_cap = 0;
     hipEvent_t gmv_done = nullptr;
     bool gmv_pending = false;
+    // pinned staging of tcme_pic_upload (grown on demand): a plane is packed
+    // there at the device pitch and sent in one copy; up_done orders reuse
+    uint8_t *h_up = nullptr;
+    size_t up_cap = 0;
+    hipEvent_t up_done = nullptr;
+    bool up_pending = false;
+    std::mutex up_mu;
     tcme::Tiling tiling;
     // plain form: the four per-size kernels run concurrently on these streams,
     // forked from and joined back into the caller's stream with events
@@ -262,7 +271,8 @@ TCME_API int tcme_create(tcme_ctx **out, int device, int max_pics)
     }
     bool ok = hipMemset(c->d_table, 0, sizeof(tcme::PicDev) * max_pics) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess &&
-              hipEventCreateWithFlags(&c->gmv_done, hipEventDisableTiming) == hipSuccess;
+              hipEventCreateWithFlags(&c->gmv_done, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&c->up_done, hipEventDisableTiming) == hipSuccess;
     for (int k = 0; k < 4 && ok; ++k)
         ok = hipStreamCreateWithFlags(&c->sub[k], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming) == hipSuccess &&
@@ -294,7 +304,9 @@ TCME_API void tcme_destroy(tcme_ctx *c)
     }
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->gmv_done) (void)hipEventDestroy(c->gmv_done);
+    if (c->up_done) (void)hipEventDestroy(c->up_done);
     (void)hipFree(c->d_gmv);
+    if (c->h_up) (void)hipHostFree(c->h_up);
     delete c;
 }
 
@@ -330,7 +342,30 @@ TCME_API int tcme_pic_upload(tcme_ctx *c, int pic, const uint8_t *host, int stri
     if (!host || stride < p.w) return fail(TCME_E_ARG, "bad host plane");
     if (set_device(c) != TCME_OK) return TCME_E_HIP;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    HIP_TRY(hipMemcpy2DAsync(p.d_y, p.pitch, host, stride, p.w, p.h, hipMemcpyHostToDevice, s));
+    // A 2-D copy from pageable memory goes one row at a time (~50 us a row);
+    // packed at the device pitch into pinned staging, the plane goes in one
+    // copy.  The caller's buffer is free again on return.
+    std::lock_guard<std::mutex> g(c->up_mu);
+    if (c->up_pending) {
+        HIP_TRY(hipEventSynchronize(c->up_done));
+        c->up_pending = false;
+    }
+    const size_t bytes = static_cast<size_t>(p.pitch) * p.h;
+    if (bytes > c->up_cap) {
+        if (c->h_up) HIP_TRY(hipHostFree(c->h_up));
+        c->h_up = nullptr;
+        c->up_cap = 0;
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&c->h_up), bytes, 0));
+        c->up_cap = bytes;
+    }
+    for (int y = 0; y < p.h; ++y) {
+        uint8_t *row = c->h_up + static_cast<size_t>(y) * p.pitch;
+        memcpy(row, host + static_cast<size_t>(y) * stride, p.w);
+        memset(row + p.w, 0, p.pitch - p.w);  // the row padding stays zero
+    }
+    HIP_TRY(hipMemcpyAsync(p.d_y, c->h_up, bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(c->up_done, s));
+    c->up_pending = true;
     return pic_derive(c, pic, s);
 }
 
@@ -696,13 +731,12 @@ struct DropIn {
     hipStream_t stream = nullptr;
     std::vector<Mirror> mirrors;
     uint64_t tick = 0;
-    int cur_pic = -1;                 // 64x64 block picture for the current block
-    uint8_t *h_stage = nullptr;       // pinned: call descriptor, then block rows
-    tcme_result *h_res = nullptr;     // pinned
-    tcme_call *d_call = nullptr;
+    // Pinned, coherent host memory that the drop-in kernel reads and writes
+    // directly: the current block (rows packed at pitch blk) and the result.
+    uint8_t *h_blk = nullptr;
+    tcme_result *h_res = nullptr;
+    const uint8_t *d_blk = nullptr;   // device addresses of the two
     tcme_result *d_res = nullptr;
-    uint8_t *d_blk = nullptr;         // luma of cur_pic
-    int blk_pitch = 0;
     long calls = 0, uploads = 0;
 };
 
@@ -721,15 +755,20 @@ void dropin_init()
     const char *dev = getenv("TCME_DEVICE");
     if (tcme_create(&d.ctx, dev ? atoi(dev) : 0, 64) != TCME_OK) die("no usable GPU");
     if (hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess) die("stream");
-    d.cur_pic = pic_alloc(d.ctx, tcme::kMaxBlk, tcme::kMaxBlk, false);
-    if (d.cur_pic < 0) die("block picture");
-    d.d_blk = d.ctx->pics[d.cur_pic].d_y;
-    d.blk_pitch = d.ctx->pics[d.cur_pic].pitch;
-    if (hipHostMalloc(reinterpret_cast<void **>(&d.h_stage), 256 + 64 * 64, 0) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&d.h_res), sizeof(tcme_result), 0) != hipSuccess ||
-        hipMalloc(&d.d_call, sizeof(tcme_call)) != hipSuccess ||
-        hipMalloc(&d.d_res, sizeof(tcme_result)) != hipSuccess)
+    // The kernel reads each 4-byte unit of the block as the aligned dword pair
+    // that covers it, so the last unit of a 64x64 block also reads bytes
+    // 4096-4099: the buffer carries the same tail slack as a device picture.
+    void *db = nullptr, *dr = nullptr;
+    if (hipHostMalloc(reinterpret_cast<void **>(&d.h_blk),
+                      tcme::kMaxBlk * tcme::kMaxBlk + tcme::kTailSlack,
+                      hipHostMallocCoherent) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&d.h_res), sizeof(tcme_result),
+                      hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer(&db, d.h_blk, 0) != hipSuccess ||
+        hipHostGetDevicePointer(&dr, d.h_res, 0) != hipSuccess)
         die("drop-in staging");
+    d.d_blk = static_cast<const uint8_t *>(db);
+    d.d_res = static_cast<tcme_result *>(dr);
 }
 
 // Device picture holding the bytes of host plane `ref` that the call reading
@@ -811,20 +850,14 @@ TCME_API tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, in
     const int rpic = (x0 <= x1 && y0 <= y1) ? mirror_for(ref, ref_stride, ref_w, ref_h, x0, y0, x1, y1)
                                             : mirror_for(ref, ref_stride, ref_w, ref_h, 0, 0, -1, -1);
 
-    tcme_call call = {rpic, d.cur_pic, 0, 0, center_x, center_y, n, search_range};
-    memcpy(d.h_stage, &call, sizeof call);
-    uint8_t *blk = d.h_stage + 256;
-    for (int r = 0; r < n; ++r) memcpy(blk + r * n, orig + static_cast<size_t>(r) * orig_stride, n);
-    if (hipMemcpyAsync(d.d_call, d.h_stage, sizeof call, hipMemcpyHostToDevice, d.stream) != hipSuccess ||
-        hipMemcpy2DAsync(d.d_blk, d.blk_pitch, blk, n, n, n, hipMemcpyHostToDevice, d.stream) != hipSuccess)
-        die("copy block");
-    const int waves = d.ctx->tiling.waves[n == 8 ? 0 : n == 16 ? 1 : n == 32 ? 2 : 3];
-    if (tcme::launch_search_n(n, waves, d.ctx->d_table, d.ctx->max_pics, d.d_call, 1, d.d_res,
-                              nullptr, d.stream) != hipSuccess)
-        die("launch");
-    if (hipMemcpyAsync(d.h_res, d.d_res, sizeof(tcme_result), hipMemcpyDeviceToHost, d.stream) != hipSuccess ||
-        hipStreamSynchronize(d.stream) != hipSuccess)
-        die("result");
+    // The call goes as a kernel argument (cur_pic and px/py are unused: the
+    // kernel reads the block from h_blk); the kernel overwrites the sentinel
+    // status, so a launch that did not run cannot pass for a result.
+    const tcme_call call = {rpic, -1, 0, 0, center_x, center_y, n, search_range};
+    for (int r = 0; r < n; ++r) memcpy(d.h_blk + r * n, orig + static_cast<size_t>(r) * orig_stride, n);
+    d.h_res->status = TCME_E_HIP;
+    if (tcme::launch_dropin(d.ctx->d_table, call, d.d_blk, d.d_res, d.stream) != hipSuccess) die("launch");
+    if (hipStreamSynchronize(d.stream) != hipSuccess) die("result");
     if (d.h_res->status != 0) {
         fail(d.h_res->status, "kernel rejected the call (status %d)", d.h_res->status);
         die("tc_motion_est");

@@ -61,6 +61,12 @@ hipError_t launch_tile(int waves, bool big, int chunk, const PicDev *table, int 
 hipError_t launch_anchors(const PicDev &p, hipStream_t s);
 // The search kernel for the calls of block size blk in the list (the others
 // are skipped); launch all four sizes to cover a mixed list.
+// One drop-in call (c.blk 8, 16, 32 or 64; c.ref_pic a reference picture of
+// `table`): `blk` and `res` are device-visible pointers, normally to mapped
+// pinned host memory; the current block is c.blk x c.blk bytes of pitch c.blk,
+// followed by at least kTailSlack readable bytes.
+hipError_t launch_dropin(const PicDev *table, const tcme_call &c, const uint8_t *blk,
+                         tcme_result *res, hipStream_t s);
 hipError_t launch_search_n(int blk, int waves, const PicDev *table, int npics,
                            const tcme_call *calls, int n, tcme_result *res, tcme_stats *stats,
                            hipStream_t s);

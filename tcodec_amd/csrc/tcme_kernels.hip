@@ -756,6 +756,24 @@ __global__ __launch_bounds__(64 * WPC) void me_search_kernel(const PicDev *__res
     search<N, WPC>(R, C, c, res + i, st, sh.win, &sh.red[0][0][0]);
 }
 
+// The per-call drop-in: one call, passed by value; the current block (N x N
+// bytes, row pitch N) is read from mapped host memory and the result written
+// to mapped host memory, so a call costs one launch and no copies.  The host
+// has checked the block size and that `ref_pic` holds a reference picture.
+template <int N, int WPC>
+__global__ __launch_bounds__(64 * WPC) void me_dropin_kernel(const PicDev *__restrict__ table,
+                                                              tcme_call c,
+                                                              const uint8_t *__restrict__ blk,
+                                                              tcme_result *__restrict__ res)
+{
+    __shared__ Shared<N, WPC> sh;
+    const PicDev R = table[c.ref_pic];
+    const PicDev C{blk, nullptr, N, N, N, 0};
+    c.px = 0;
+    c.py = 0;
+    search<N, WPC>(R, C, c, res, nullptr, sh.win, &sh.red[0][0][0]);
+}
+
 // Sum over each group of L consecutive lanes (L = 1, 2, 4, ..., 64, groups
 // aligned to L); every lane of a group gets its group's total.  DPP
 // butterflies: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
@@ -1460,6 +1478,28 @@ hipError_t launch_tile(int waves, bool big, int chunk, const PicDev *table, int 
     else
         hipLaunchKernelGGL((me_tile_kernel<8, false>), dim3(grid), dim3(512), 0, s, table, npics,
                            calls, n, chunk, res, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_dropin(const PicDev *table, const tcme_call &c, const uint8_t *blk,
+                         tcme_result *res, hipStream_t s)
+{
+    switch (c.blk) {
+    case 8:
+        hipLaunchKernelGGL((me_dropin_kernel<8, 1>), dim3(1), dim3(64), 0, s, table, c, blk, res);
+        break;
+    case 16:
+        hipLaunchKernelGGL((me_dropin_kernel<16, 1>), dim3(1), dim3(64), 0, s, table, c, blk, res);
+        break;
+    case 32:
+        hipLaunchKernelGGL((me_dropin_kernel<32, 4>), dim3(1), dim3(256), 0, s, table, c, blk, res);
+        break;
+    case 64:
+        hipLaunchKernelGGL((me_dropin_kernel<64, 8>), dim3(1), dim3(512), 0, s, table, c, blk, res);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

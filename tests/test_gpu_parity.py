@@ -191,6 +191,10 @@ def test_dropin_symbol_matches_oracle(orc):
             blk = np.ascontiguousarray(cur[py:py + 8, px:px + 8])
             (mx, my), sad = me.motion_est(rbuf, stride, w, h, blk, 0, 8, cx, cy, 8, sr)
             assert (mx, my, sad) == orc.motion_est(ref, cur, px, py, cx, cy, 8, sr)
+        # a 64x64 block: its last 4-byte unit is read as the aligned dword pair
+        # that covers it, 4 bytes past the block in the drop-in's pinned buffer
+        (mx, my), sad = me.motion_est(rbuf, stride, w, h, cur, 16 * w + 40, w, 37, 21, 64, 32)
+        assert (mx, my, sad) == orc.motion_est(ref, cur, 40, 16, 37, 21, 64, 32)
         # same buffer, new content: next calls must see it
         rbuf[:, :w] = np.roll(ref, 5, 1)
         new = np.ascontiguousarray(rbuf[:, :w])
