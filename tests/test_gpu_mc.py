@@ -43,6 +43,37 @@ def test_gpu_luma_mc_matches_oracle(engine, orc, shape):
     engine.free(rp)
 
 
+def test_gpu_luma_mc_interior_blocks(engine, orc):
+    """Blocks whose every sample has its 6x6 apron inside the frame (the
+    kernel's batched path), each of the 16 phases, block sizes that do and do
+    not divide a wavefront, next to blocks on the edges of that region."""
+    from tcodec_amd import synth
+    print("luma mc interior 640x360", flush=True)
+    h, w = 360, 640
+    ref = synth.Scene(w, h, seed=11, rects=12).frame(0)[0]
+    rp = engine.pic(ref)
+    rng = np.random.default_rng(5)
+    count = 3000
+    sizes = rng.choice([8, 16, 32, 64, 12, 24, 48], count)
+    fx = np.array([rng.integers(2, w - n - 2) for n in sizes])
+    fy = np.array([rng.integers(2, h - n - 2) for n in sizes])
+    mvs = np.stack([fx * 4 + rng.integers(0, 4, count), fy * 4 + rng.integers(0, 4, count)], 1)
+    mvs[:16] = np.stack([fx[:16] * 4 + np.arange(16) % 4, fy[:16] * 4 + np.arange(16) // 4], 1)
+    # the last interior positions and one sample beyond them
+    edge = count - 8
+    mvs[edge:edge + 4, 0] = (w - sizes[edge:edge + 4] - 3 + np.arange(4) % 2) * 4 + 3
+    mvs[edge + 4:, 1] = (h - sizes[edge + 4:] - 3 + np.arange(4) % 2) * 4 + 3
+    preds = _preds(rp, mvs, sizes)
+    out, st = engine.predict(preds)
+    assert (st == 0).all()
+    for k in range(count):
+        n = int(sizes[k])
+        o = int(preds["out_off"][k])
+        want = orc.inter_predict(ref, int(mvs[k, 0]), int(mvs[k, 1]), n)
+        assert np.array_equal(out[o:o + n * n].reshape(n, n), want), (k, mvs[k], n)
+    engine.free(rp)
+
+
 @pytest.mark.parametrize("shape", [(30, 48), (64, 80), (20, 33)])
 def test_gpu_chroma_mc_matches_oracle(engine, orc, shape):
     h, w = shape
