@@ -74,6 +74,45 @@ def test_gpu_luma_mc_interior_blocks(engine, orc):
     engine.free(rp)
 
 
+def test_gpu_uploads_back_to_back(engine):
+    """Two uploads in a row from one host buffer that is rewritten in between,
+    read through a row stride wider than the plane (and a device pitch wider
+    still): each picture holds exactly the bytes of its own upload.  Read back
+    by integer-MV chroma prediction, which returns the plane's samples with the
+    coordinates clamped to the frame."""
+    import ctypes
+
+    from tcodec_amd import me
+    rng = np.random.default_rng(17)
+    h, w = 100, 136
+    wide = rng.integers(0, 256, (h, 200), dtype=np.uint8)
+    pa = engine.plane(np.zeros((h, w), np.uint8))
+    pb = engine.plane(np.zeros((h, w), np.uint8))
+
+    def upload(pid):  # the buffer itself, row stride 200 (no contiguous copy)
+        assert engine.L.tcme_pic_upload(engine.h, pid, wide.ctypes.data_as(me.u8p), 200,
+                                        ctypes.c_void_p(0)) == 0
+
+    upload(pa)
+    first = wide[:, :w].copy()
+    wide[:] = 255 - wide
+    upload(pb)
+    second = wide[:, :w].copy()
+    xs, ys = np.meshgrid(np.arange(-8, w + 8, 16), np.arange(-8, h + 8, 16))
+    mvs = np.stack([xs.ravel() * 8, ys.ravel() * 8], 1)
+    sizes = np.full(len(mvs), 16)
+    for pid, want in ((pa, first), (pb, second)):
+        out, st = engine.predict(_preds(pid, mvs, sizes), chroma=True)
+        assert (st == 0).all()
+        for k, (x, y) in enumerate(zip(xs.ravel(), ys.ravel())):
+            rows = np.clip(np.arange(y, y + 16), 0, h - 1)
+            cols = np.clip(np.arange(x, x + 16), 0, w - 1)
+            got = out[k * 256:(k + 1) * 256].reshape(16, 16)
+            assert np.array_equal(got, want[np.ix_(rows, cols)]), (pid, x, y)
+    engine.free(pa)
+    engine.free(pb)
+
+
 @pytest.mark.parametrize("shape", [(30, 48), (64, 80), (20, 33)])
 def test_gpu_chroma_mc_matches_oracle(engine, orc, shape):
     h, w = shape
