@@ -252,12 +252,16 @@ def dropin_rate(frames, scene, pics, samples: int = 400):
 
 
 def encoder_leg():
-    """The reference encoder (oracle/_ref/tcenc, built from its sources) on the
-    CPU-runnable configuration - 320x240, 8 frames of the diagonal-gradient
-    source (+3,-2) px per frame with noise, v2, P-only, one reference, QP 32 -
-    and the same encoder linked to the drop-in (oracle/_ref/tcenc_gpu): wall
-    time of each process (the GPU one includes its HIP start-up) and whether
-    the .tcv bytes are identical."""
+    """The reference encoder on the CPU-runnable configuration - 320x240, 8
+    frames of the diagonal-gradient source (+3,-2) px per frame with noise,
+    v2, P-only, one reference, QP 32 - built from its sources (oracle/_ref/tcenc)
+    and linked to the drop-in (oracle/_ref/tcenc_gpu): each encoder's own clock
+    ("Time: N sec", which leaves out process start-up but, for the drop-in,
+    includes the GPU runtime's start in the first search) and process wall
+    time, and whether the .tcv bytes are identical.  Then the same encoder with
+    each tc_motion_est call timed (CLOCK_MONOTONIC around the call,
+    oracle/_ref/tcenc_hook): the reference search, then the drop-in, per call
+    after the first, and the drop-in's first call (runtime start) on its own."""
     import tempfile
 
     from tcodec_amd import synth
@@ -265,7 +269,10 @@ def encoder_leg():
     exes = {"cpu": os.path.join(ref, "tcenc"), "gpu_dropin": os.path.join(ref, "tcenc_gpu")}
     if not all(os.path.exists(e) for e in exes.values()):
         return None
-    out = {"clip": "320x240, 8 frames, diagonal gradient + noise, v2, P-only, 1 reference, QP 32"}
+    args = ["-w", "320", "-h", "240", "--v2", "-q", "32"]
+    enc_clock = re.compile(r"Time:\s*([0-9.]+)\s*sec")
+    out = {"clip": "320x240, 8 frames, diagonal gradient + noise, v2, P-only, 1 reference, QP 32",
+           "cores": 1}
     with tempfile.TemporaryDirectory() as td:
         yuv = os.path.join(td, "in.yuv")
         synth.write_yuv(yuv, synth.gradient_video(320, 240, 8, seed=1))
@@ -273,18 +280,38 @@ def encoder_leg():
         for name, exe in exes.items():
             tcv[name] = os.path.join(td, name + ".tcv")
             t0 = time.perf_counter()
-            r = subprocess.run([exe, "-w", "320", "-h", "240", "--v2", "-q", "32", "-o", tcv[name], yuv],
+            r = subprocess.run([exe, *args, "-o", tcv[name], yuv],
                                check=True, capture_output=True, text=True, timeout=300)
-            dt = time.perf_counter() - t0
-            # the encoder's own clock (its "Time: N sec" summary line) leaves out
-            # process and HIP start-up; the wall time includes them
-            m = re.search(r"Time:\s*([0-9.]+)\s*sec", r.stdout + r.stderr)
+            wall = time.perf_counter() - t0
+            m = enc_clock.search(r.stdout + r.stderr)
             enc = float(m.group(1)) if m else None
-            out[name + "_wall_s"] = round(dt, 3)
-            out[name + "_encode_s"] = enc
-            out[name + "_frames_per_s"] = round(8 / enc, 2) if enc else None
+            out[name] = {"encode_s": enc, "frames_per_s": round(8 / enc, 2) if enc else None,
+                         "wall_s": round(wall, 3)}
         out["tcv_identical"] = open(tcv["cpu"], "rb").read() == open(tcv["gpu_dropin"], "rb").read()
-    out["cores"] = 1
+        hook = os.path.join(ref, "tcenc_hook")
+        lib = os.path.join(ROOT, "tcodec_amd", "lib", "libtcme.so")
+        if not os.path.exists(hook):
+            return out
+        shim = re.compile(r"calls=(\d+) mismatches=\d+ me_seconds=([0-9.]+) "
+                          r"first_call_seconds=([0-9.]+)")
+        for mode, name in (("ref", "cpu"), ("gpu", "gpu_dropin")):
+            env = dict(os.environ, TCME_SHIM=mode, TCME_LIB=lib)
+            r = subprocess.run([hook, *args, "-o", os.path.join(td, mode + ".tcv"), yuv],
+                               check=True, capture_output=True, text=True, timeout=300, env=env)
+            m, e = shim.search(r.stderr), enc_clock.search(r.stdout + r.stderr)
+            if not m:
+                continue
+            calls, sec, first = int(m.group(1)), float(m.group(2)), float(m.group(3))
+            d = out[name]
+            out["me_calls"] = calls
+            d["me_s"] = round(sec, 4)
+            d["me_us_per_call"] = round((sec - first) / max(calls - 1, 1) * 1e6, 2)
+            if e:
+                d["me_share_of_encode"] = round(sec / float(e.group(1)), 3)
+            if mode == "gpu":
+                d["first_call_s"] = round(first, 4)
+                if e:
+                    d["encode_s_after_first_call"] = round(float(e.group(1)) - first, 3)
     return out
 
 
