@@ -119,7 +119,9 @@ int  tcme_search(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *resu
  *                    CTU and reference (85 calls per group) suit chunk 85.
  *                    0: no tile workgroups.
  *   tile_big         1 (with tile_waves 4 or 8): the tile workgroups take
- *                    every call, all block sizes (one launch) */
+ *                    every call, all block sizes (one launch)
+ * A new context starts with {1, 1, 1, 4, 8, 85, 1}: every call in one launch
+ * of 8-wavefront tile workgroups, 85 calls each. */
 typedef struct tcme_tiling {
     int32_t waves8, waves16, waves32, waves64;
     int32_t tile_waves, tile_chunk;
