@@ -9,7 +9,13 @@ block position, search range 32 (B) / 24 (P).  References are the source
 frames (open loop).  Pictures are resident in HBM before timing starts; a
 step is one launch per frame on one stream.
 
-Prints one JSON line (see the repo's measurement notes for field meanings).
+Prints one JSON line: `value` is ME frames/s of the timed steps; `roofline`
+the dominant kernel's algorithmic bytes per launch over its average launch
+time (HIP events on the launch stream) against 8 TB/s, with `traffic` its HBM
+bytes per launch from the committed counter summary; `cpu_baseline` the
+reference's own search on the host cores; the other fields are the further
+legs (drop-in per call, motion compensation, global motion and SADs, the
+reference encoder with and without the drop-in, batched and served forms).
 """
 from __future__ import annotations
 
@@ -430,7 +436,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by stream/frame band, see the design notes)", file=sys.stderr)
+              "(it would shard by whole streams, or by bands of CTU rows with a search-range "
+              "apron, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
