@@ -421,7 +421,9 @@ def main():
                     help="first copy this many MiB device-to-device (counter calibration)")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the batched (64 streams) and served (Poisson) runs")
-    ap.add_argument("--batch-steps", type=int, default=4)
+    ap.add_argument("--batch-steps", type=int, default=55,
+                    help="timed time steps of the batched form (55 + 5 warm-up = the "
+                         "configuration's 60 pictures per stream)")
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-seconds", type=float, default=2.0,
                     help="arrival window of the served run (the serving config uses 120)")
