@@ -29,6 +29,11 @@ CASES = {
                                      ["--v2", "--profile", "1", "-p", "2", "-q", "27"]),
     "scene_480x272_bgop4_q32": ("scene", 480, 272, 9,
                                 ["--v2", "--profile", "1", "-p", "2", "-q", "32", "-b"]),
+    # the 720p and 1080p configurations' sizes and flags, on short clips
+    "scene_1280x720_p_multiref_q27": ("scene", 1280, 720, 4,
+                                      ["--v2", "--profile", "1", "-p", "2", "-q", "27"]),
+    "scene_1920x1080_bgop4_q32": ("scene", 1920, 1080, 5,
+                                  ["--v2", "--profile", "1", "-p", "2", "-q", "32", "-b"]),
 }
 
 
