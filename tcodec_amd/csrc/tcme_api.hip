@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -855,15 +857,30 @@ TCME_API tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, in
     // status, so a launch that did not run cannot pass for a result.
     const tcme_call call = {rpic, -1, 0, 0, center_x, center_y, n, search_range};
     for (int r = 0; r < n; ++r) memcpy(d.h_blk + r * n, orig + static_cast<size_t>(r) * orig_stride, n);
-    d.h_res->status = TCME_E_HIP;
+    volatile tcme_result *hr = d.h_res;
+    hr->status = TCME_E_HIP;
     if (tcme::launch_dropin(d.ctx->d_table, call, d.d_blk, d.d_res, d.stream) != hipSuccess) die("launch");
-    if (hipStreamSynchronize(d.stream) != hipSuccess) die("result");
-    if (d.h_res->status != 0) {
-        fail(d.h_res->status, "kernel rejected the call (status %d)", d.h_res->status);
+    // The kernel writes the status last, after a system-scope fence: poll it
+    // (a blocking synchronisation adds a wake-up to every call).  Past 2 ms
+    // without it, synchronise, which also reports a failed launch.  The kernel
+    // reads the block only before it writes the result, so the next call may
+    // refill it while this kernel is still retiring.
+    const auto t0 = std::chrono::steady_clock::now();
+    bool seen = false;
+    for (unsigned spin = 1;; ++spin) {
+        if (hr->status != TCME_E_HIP) { seen = true; break; }
+        if ((spin & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2))
+            break;
+        __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    if (!seen && hipStreamSynchronize(d.stream) != hipSuccess) die("result");
+    if (hr->status != 0) {
+        fail(hr->status, "kernel rejected the call (status %d)", hr->status);
         die("tc_motion_est");
     }
     d.calls++;
-    if (best_sad_out) *best_sad_out = d.h_res->sad;
-    tcme_mv mv = {d.h_res->mv_x, d.h_res->mv_y};
+    if (best_sad_out) *best_sad_out = hr->sad;
+    tcme_mv mv = {hr->mv_x, hr->mv_y};
     return mv;
 }

@@ -758,8 +758,10 @@ __global__ __launch_bounds__(64 * WPC) void me_search_kernel(const PicDev *__res
 
 // The per-call drop-in: one call, passed by value; the current block (N x N
 // bytes, row pitch N) is read from mapped host memory and the result written
-// to mapped host memory, so a call costs one launch and no copies.  The host
-// has checked the block size and that `ref_pic` holds a reference picture.
+// to mapped host memory, status last, so a call costs one launch and no
+// copies, and the host can poll the status instead of synchronising.  The
+// host has checked the block size and that `ref_pic` holds a reference
+// picture.
 template <int N, int WPC>
 __global__ __launch_bounds__(64 * WPC) void me_dropin_kernel(const PicDev *__restrict__ table,
                                                               tcme_call c,
@@ -767,11 +769,22 @@ __global__ __launch_bounds__(64 * WPC) void me_dropin_kernel(const PicDev *__res
                                                               tcme_result *__restrict__ res)
 {
     __shared__ Shared<N, WPC> sh;
+    __shared__ tcme_result r;
     const PicDev R = table[c.ref_pic];
     const PicDev C{blk, nullptr, N, N, N, 0};
     c.px = 0;
     c.py = 0;
-    search<N, WPC>(R, C, c, res, nullptr, sh.win, &sh.red[0][0][0]);
+    search<N, WPC>(R, C, c, &r, nullptr, sh.win, &sh.red[0][0][0]);
+    __syncthreads();
+    // the host polls the status: it goes last, after the other fields are
+    // visible system-wide
+    if (threadIdx.x == 0) {
+        res->mv_x = r.mv_x;
+        res->mv_y = r.mv_y;
+        res->sad = r.sad;
+        __threadfence_system();
+        res->status = r.status;
+    }
 }
 
 // Sum over each group of L consecutive lanes (L = 1, 2, 4, ..., 64, groups
