@@ -11,7 +11,7 @@
 // result is always computed from exactly the bytes the caller passed, however
 // the caller reuses its buffers.  The call itself is a kernel argument; the
 // kernel reads the current block from, and writes the result to, pinned host
-// memory, so a call is one launch and one synchronisation.
+// memory, status last, so a call is one launch and a poll of that word.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
