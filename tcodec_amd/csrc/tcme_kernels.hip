@@ -1407,43 +1407,80 @@ __global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile
 }
 
 // Anchor plane of a picture (reference src/motion.c:70-107 for the taps):
-// one thread per sample writes {sample, hh, vv, cc}; half samples whose taps
-// would leave the frame are written as 0 and never read by the search.
+// {sample, hh, vv, cc} per sample; half samples whose taps would leave the
+// frame are written as 0 and never read by the search.  A workgroup of 64 x 4
+// threads makes a tile of kAncW x kAncH samples: it stages the tile's luma
+// with its apron (2 left / up, 3 right / down; coordinates clamped into the
+// frame, and the clamped values never reach a written half sample) in LDS,
+// takes the raw horizontal six-tap of every staged row once, and builds the
+// diagonal half sample as the vertical six-tap of those (the reference's hv
+// order, src/motion.c:94-107).  A thread writes kAncH / 4 rows of one column:
+// every row store of a wavefront is 512 contiguous bytes.
+constexpr int kAncW = 64, kAncH = 16;
 __global__ __launch_bounds__(256) void anchor_kernel(const uint8_t *__restrict__ y, int pitch,
                                                      int w, int h, uint64_t *__restrict__ an,
                                                      int apitch)
 {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int r = blockIdx.y * 4 + threadIdx.y;
-    if (x >= w || r >= h) return;
-    const bool okx = x >= 2 && x + 3 < w;
-    const bool oky = r >= 2 && r + 3 < h;
-    int hv = 0, vval = 0, cval = 0;
-    const uint8_t *row = y + static_cast<size_t>(r) * pitch;
-    if (okx) hv = tap6(row[x - 2], row[x - 1], row[x], row[x + 1], row[x + 2], row[x + 3]);
-    if (oky) {
-        const uint8_t *col = row + x;
-        vval = tap6(col[-2 * pitch], col[-pitch], col[0], col[pitch], col[2 * pitch], col[3 * pitch]);
-    }
-    if (okx && oky) {
-        int t[6];
+    constexpr int SR = kAncH + 5, SC = kAncW + 8;  // staged rows / columns
+    __shared__ uint8_t lum[SR][SC];                // row y0 - 2 + r, column x0 - 2 + c
+    __shared__ int16_t hs[SR][kAncW];              // horizontal six-tap at column x0 + c
+    const int x0 = static_cast<int>(blockIdx.x) * kAncW, y0 = static_cast<int>(blockIdx.y) * kAncH;
+    const int tx = static_cast<int>(threadIdx.x), ty = static_cast<int>(threadIdx.y);
+    const int tid = ty * 64 + tx;
+    // every load issued before the first LDS store (a rolled loop waited for
+    // each one); slots past the staged area re-read its last sample
+    constexpr int NS = (SR * SC + 255) / 256;
+    uint8_t v[NS];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const uint8_t *rr = y + static_cast<size_t>(r - 2 + k) * pitch + x;
-            t[k] = tap6(rr[-2], rr[-1], rr[0], rr[1], rr[2], rr[3]);
-        }
-        cval = tap6(t[0], t[1], t[2], t[3], t[4], t[5]);
+    for (int k = 0; k < NS; ++k) {
+        const int i = min(tid + 256 * k, SR * SC - 1);
+        const int r = i / SC, c = i - r * SC;
+        const int gy = min(max(y0 - 2 + r, 0), h - 1), gx = min(max(x0 - 2 + c, 0), w - 1);
+        v[k] = y[static_cast<size_t>(gy) * pitch + gx];
     }
-    const uint32_t lo = static_cast<uint32_t>(row[x]) | (static_cast<uint32_t>(hv) << 16);
-    const uint32_t hi = (static_cast<uint32_t>(vval) & 0xFFFFu) | (static_cast<uint32_t>(cval) << 16);
-    an[static_cast<size_t>(r) * apitch + x] = static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int i = tid + 256 * k;
+        if (i < SR * SC) lum[i / SC][i % SC] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < (SR * kAncW + 255) / 256; ++k) {
+        const int i = tid + 256 * k;
+        if (i >= SR * kAncW) break;
+        const int r = i / kAncW, c = i - r * kAncW;
+        const uint8_t *q = &lum[r][c];  // columns x - 2 .. x + 3
+        hs[r][c] = static_cast<int16_t>(tap6(q[0], q[1], q[2], q[3], q[4], q[5]));
+    }
+    __syncthreads();
+    const int x = x0 + tx;
+    if (x >= w) return;
+    const bool okx = x >= 2 && x + 3 < w;
+#pragma unroll
+    for (int k = 0; k < kAncH / 4; ++k) {
+        const int t = ty + 4 * k;  // tile row; staged rows t .. t + 5 are image rows r - 2 .. r + 3
+        const int r = y0 + t;
+        if (r >= h) break;
+        const bool oky = r >= 2 && r + 3 < h;
+        const int hh = okx ? hs[t + 2][tx] : 0;
+        int vv = 0, cc = 0;
+        if (oky) {
+            vv = tap6(lum[t][tx + 2], lum[t + 1][tx + 2], lum[t + 2][tx + 2], lum[t + 3][tx + 2],
+                      lum[t + 4][tx + 2], lum[t + 5][tx + 2]);
+            if (okx) cc = tap6(hs[t][tx], hs[t + 1][tx], hs[t + 2][tx], hs[t + 3][tx], hs[t + 4][tx],
+                               hs[t + 5][tx]);
+        }
+        const uint32_t lo = static_cast<uint32_t>(lum[t + 2][tx + 2]) | (static_cast<uint32_t>(hh) << 16);
+        const uint32_t hi = (static_cast<uint32_t>(vv) & 0xFFFFu) | (static_cast<uint32_t>(cc) << 16);
+        an[static_cast<size_t>(r) * apitch + x] = static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+    }
 }
 
 }  // namespace
 
 hipError_t launch_anchors(const PicDev &p, hipStream_t s)
 {
-    dim3 grid((p.w + 63) / 64, (p.h + 3) / 4), block(64, 4);
+    dim3 grid((p.w + kAncW - 1) / kAncW, (p.h + kAncH - 1) / kAncH), block(64, 4);
     hipLaunchKernelGGL(anchor_kernel, grid, block, 0, s, p.y, p.pitch, p.w, p.h,
                        const_cast<uint64_t *>(p.an), p.apitch);
     return hipGetLastError();

@@ -74,6 +74,27 @@ def test_gpu_luma_mc_interior_blocks(engine, orc):
     engine.free(rp)
 
 
+def test_gpu_luma_mc_every_position_and_phase(engine, orc):
+    """1x1 predictions at every position of a 70x37 frame (two anchor tiles
+    across, three down) at each of the 16 quarter-sample phases: an interior
+    position reads the anchor fields (sample, hh, vv, cc) its phase selects,
+    an edge one the bilinear or nearest ladder, so every anchor the plane
+    holds is checked against the oracle's tc_inter_predict."""
+    h, w = 37, 70
+    print(f"luma mc every position and phase {w}x{h}", flush=True)
+    ref = np.random.default_rng(23).integers(0, 256, (h, w), dtype=np.uint8)
+    rp = engine.pic(ref)
+    ys, xs, ph = np.meshgrid(np.arange(h), np.arange(w), np.arange(16), indexing="ij")
+    mvs = np.stack([xs.ravel() * 4 + ph.ravel() % 4, ys.ravel() * 4 + ph.ravel() // 4], 1)
+    preds = _preds(rp, mvs, np.ones(len(mvs), np.int64))
+    out, st = engine.predict(preds)
+    assert (st == 0).all()
+    want = np.array([orc.inter_predict(ref, int(mx), int(my), 1)[0, 0] for mx, my in mvs])
+    bad = np.nonzero(out[:len(mvs)] != want)[0]
+    assert bad.size == 0, [(tuple(mvs[k]), int(out[k]), int(want[k])) for k in bad[:8]]
+    engine.free(rp)
+
+
 def test_gpu_uploads_back_to_back(engine):
     """Two uploads in a row from one host buffer that is rewritten in between,
     read through a row stride wider than the plane (and a device pitch wider
