@@ -360,8 +360,9 @@ def mc_leg(eng, frames, d_res, reps: int = 5):
 
 def aux_leg(eng, frames, pics, reps: int = 3):
     """Global-motion estimates for every timed picture against its nearest
-    reference, and the co-located 8x8 and 64x64 stability SADs of one picture
-    against the previous one, each as one launch."""
+    reference, the co-located 8x8 and 64x64 stability SADs of one picture
+    against the previous one, and one picture's anchor plane (built once per
+    reference picture), each as one launch."""
     import torch
     from tcodec_amd import me
     pairs = np.array([(int(c["cur_pic"][0]), int(np.asarray(c["ref_pic"])[0]))
@@ -388,8 +389,11 @@ def aux_leg(eng, frames, pics, reps: int = 3):
         me._check(eng.L.tcme_sad(eng.h, me.C.c_void_p(d_d.data_ptr()), len(d), None, 0,
                                  me.C.c_void_p(d_s.data_ptr()), me.C.c_void_p(s.cuda_stream)))
 
+    def anchors():  # rebuilds one picture's anchor plane from its (unchanged) luma
+        me._check(eng.L.tcme_pic_finish(eng.h, cur, me.C.c_void_p(s.cuda_stream)))
+
     out = {}
-    for name, fn in (("global_mv", gmv), ("stability_sad", sads)):
+    for name, fn in (("global_mv", gmv), ("stability_sad", sads), ("anchors", anchors)):
         fn()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(s)
@@ -397,12 +401,17 @@ def aux_leg(eng, frames, pics, reps: int = 3):
             fn()
         e1.record(s)
         torch.cuda.synchronize()
-        out[name] = round(e0.elapsed_time(e1) / reps, 3)
+        out[name] = round(e0.elapsed_time(e1) / reps, 4)
     assert (d_out[:, 2] == 0).all().item() and (d_s >= 0).all().item()
     res = {"global_mv": {"pairs": len(pairs), "ms": out["global_mv"],
                          "ms_per_pair": round(out["global_mv"] / len(pairs), 3)},
            "stability_sad": {"blocks": len(d), "ms": out["stability_sad"],
-                             "GBs": round(2 * W * H * 2 / out["stability_sad"] / 1e6, 1)}}
+                             "GBs": round(2 * W * H * 2 / out["stability_sad"] / 1e6, 1)},
+           # luma read once, one 8-byte anchor written per sample
+           "anchors": {"kernel": "anchor_kernel", "pictures": "1920x1080, once per reference",
+                       "ms": out["anchors"], "bytes": W * H * 9,
+                       "GBs": round(W * H * 9 / out["anchors"] / 1e6, 1),
+                       "frac": round(W * H * 9 / out["anchors"] / 1e6 / HBM_PEAK_GBS, 4)}}
     log(f"aux: {res}")
     return res
 
