@@ -223,15 +223,43 @@ __global__ __launch_bounds__(kGmvThreads) void gmv_walk_kernel(const PicDev *__r
 
 // One wavefront per descriptor, the wavefronts striding over the list; lanes
 // take samples, a DPP scan sums them.
+// |a - b| summed over rows r0, r0 + dr, ... < h of one column, K rows per lane
+// at a time with every load issued first (rows past the block re-read its
+// last row and are not counted).
+template <int K>
+__device__ __forceinline__ int sad_rows(gptr8 a, int ap, gptr8 b, int bp, int r0, int dr, int h)
+{
+    int s = 0;
+    for (int rb = r0; rb < h; rb += K * dr) {
+        int pa[K], pb[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int r = min(rb + k * dr, h - 1);
+            pa[k] = a[static_cast<size_t>(r) * ap];
+            pb[k] = b[static_cast<size_t>(r) * bp];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int d = pa[k] - pb[k];
+            if (rb + k * dr < h) s += d < 0 ? -d : d;
+        }
+    }
+    return s;
+}
+
+// One wavefront per block, the wavefronts striding over the list; the block
+// index is wave-uniform, so descriptor and picture-table entries are scalar
+// loads.  Lane l takes column l mod w of rows l / w, l / w + 64 / w, ...
+// (lanes past the last whole row idle), up to 8 rows at a time.
 __global__ __launch_bounds__(256) void sad_kernel(const PicDev *__restrict__ table, int npics,
                                                   const tcme_sad_desc *__restrict__ descs, int n,
                                                   const uint8_t *__restrict__ buf,
                                                   int64_t buf_bytes, int32_t *__restrict__ out)
 {
     const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     const int stride = static_cast<int>(gridDim.x) * 4;
-    for (int i = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6); i < n;
-         i += stride) {
+    for (int i = static_cast<int>(blockIdx.x) * 4 + wave; i < n; i += stride) {
         const tcme_sad_desc d = descs[i];
         int status = 0;
         PicDev A{}, B{};
@@ -259,11 +287,13 @@ __global__ __launch_bounds__(256) void sad_kernel(const PicDev *__restrict__ tab
         const gptr8 b = d.other_pic >= 0 ? (gptr8)B.y + static_cast<size_t>(d.oy) * B.pitch + d.ox
                                          : (gptr8)buf + d.buf_off;
         const int bp = d.other_pic >= 0 ? B.pitch : d.buf_stride;
+        const int dr = 64 / d.w, c = lane % d.w, r0 = lane / d.w;
+        const int rows = (d.h + dr - 1) / dr;  // per lane; wave-uniform
         int s = 0;
-        for (int k = lane; k < d.w * d.h; k += 64) {
-            const int r = k / d.w, c = k - r * d.w;
-            const int p = a[static_cast<size_t>(r) * A.pitch + c], q = b[static_cast<size_t>(r) * bp + c];
-            s += p > q ? p - q : q - p;
+        if (r0 < dr) {
+            if (rows >= 8) s = sad_rows<8>(a + c, A.pitch, b + c, bp, r0, dr, d.h);
+            else if (rows >= 4) s = sad_rows<4>(a + c, A.pitch, b + c, bp, r0, dr, d.h);
+            else s = sad_rows<1>(a + c, A.pitch, b + c, bp, r0, dr, d.h);
         }
         s += __builtin_amdgcn_update_dpp(0, s, 0x111, 0xF, 0xF, false);
         s += __builtin_amdgcn_update_dpp(0, s, 0x112, 0xF, 0xF, false);
