@@ -321,9 +321,11 @@ def encoder_leg():
     return out
 
 
-def mc_leg(eng, frames, d_res, reps: int = 5):
+def mc_leg(eng, frames, d_res, scene=None, pics=None, reps: int = 5):
     """Luma motion compensation of every block of the first frame at its own
-    search result (the consumer of each search): one launch, blocks packed."""
+    search result (the consumer of each search): one launch, blocks packed;
+    then, with the scene, both 4:2:0 chroma planes of every block at the same
+    result (the luma quarter-pel MV read as 1/8 chroma sample), one launch."""
     import torch
     from tcodec_amd import me
     poc, kind, calls = frames[0]
@@ -354,6 +356,42 @@ def mc_leg(eng, frames, d_res, reps: int = 5):
            "ms": round(ms, 3), "Gpixels_per_s": round(nbytes / ms / 1e6, 2),
            "algo_bytes": algo, "achieved_GBs": round(algo / ms / 1e6, 1),
            "frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    if scene is not None:
+        poc_of = {pid: t for t, pid in pics.items()}
+        planes = {}
+        for rp in sorted(set(calls["ref_pic"].tolist())):
+            _, cb, cr = scene.frame(poc_of[rp])
+            planes[rp] = (eng.plane(cb), eng.plane(cr))
+        m = n // 2
+        cp = np.zeros(2 * len(calls), me.PRED_DTYPE)
+        for k in (0, 1):
+            part = cp[k * len(calls):(k + 1) * len(calls)]
+            part["pic"] = [planes[int(r)][k] for r in calls["ref_pic"]]
+            part["mv_x"], part["mv_y"] = res[:, 0], res[:, 1]
+            part["blk"] = m
+            part["out_stride"] = m
+        mm = np.concatenate([m, m])
+        cp["out_off"] = np.concatenate([[0], np.cumsum(mm * mm)[:-1]])
+        cbytes = int((mm * mm).sum())
+        d_cp = torch.from_numpy(cp.view(np.uint8).copy()).cuda()
+        d_co = torch.empty(cbytes, dtype=torch.uint8, device="cuda")
+        eng.predict_device(d_cp.data_ptr(), len(cp), d_co.data_ptr(), cbytes, None, True,
+                           s.cuda_stream)
+        e0.record(s)
+        for _ in range(reps):
+            eng.predict_device(d_cp.data_ptr(), len(cp), d_co.data_ptr(), cbytes, None, True,
+                               s.cuda_stream)
+        e1.record(s)
+        torch.cuda.synchronize()
+        cms = e0.elapsed_time(e1) / reps
+        calgo = int(((mm + 1) ** 2 + mm * mm).sum())   # bilinear apron read + block written
+        out["chroma"] = {"kernel": "mc_chroma_kernel", "blocks": len(cp), "pixels": cbytes,
+                         "ms": round(cms, 3), "Gpixels_per_s": round(cbytes / cms / 1e6, 2),
+                         "algo_bytes": calgo, "achieved_GBs": round(calgo / cms / 1e6, 1),
+                         "frac": round(calgo / cms / 1e6 / HBM_PEAK_GBS, 4)}
+        for pa, pb in planes.values():
+            eng.free(pa)
+            eng.free(pb)
     log(f"mc: {out}")
     return out
 
@@ -583,7 +621,7 @@ def main():
     log("kernel ms per frame: " + ", ".join(f"{n} {t:.3f}" for (n, _), t in zip(ks, kt)))
 
     cpu = None if args.no_cpu else cpu_baseline(scene, pics, frames, d_res[0].cpu().numpy())
-    mc = mc_leg(eng, frames, d_res)
+    mc = mc_leg(eng, frames, d_res, scene, pics)
     aux = aux_leg(eng, frames, pics)
     try:
         encoder = None if args.no_cpu else encoder_leg()
