@@ -276,8 +276,44 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
     }
 }
 
-// Chroma at 1/8 sample: integer mv >> 3, fraction mv & 7, edge-clamped
-// bilinear ((8-fx)(8-fy)A + fx(8-fy)B + (8-fx)fy C + fx fy D + 32) >> 6.
+// Chroma sample (c, r) of a block at 1/8-sample MV (integer part xi, yi,
+// fraction fx, fy): edge-clamped bilinear
+// ((8-fx)(8-fy)A + fx(8-fy)B + (8-fx)fy C + fx fy D + 32) >> 6.
+__device__ __forceinline__ int chroma_px(const PicDev &R, int xi, int yi, int fx, int fy, int c,
+                                         int r)
+{
+    const gptr8 Y = (gptr8)R.y;
+    const int x0 = min(max(xi + c, 0), R.w - 1), x1 = min(max(xi + c + 1, 0), R.w - 1);
+    const int y0 = min(max(yi + r, 0), R.h - 1), y1 = min(max(yi + r + 1, 0), R.h - 1);
+    const int a = Y[static_cast<size_t>(y0) * R.pitch + x0];
+    const int b = Y[static_cast<size_t>(y0) * R.pitch + x1];
+    const int cc = Y[static_cast<size_t>(y1) * R.pitch + x0];
+    const int d = Y[static_cast<size_t>(y1) * R.pitch + x1];
+    return ((8 - fx) * (8 - fy) * a + fx * (8 - fy) * b + (8 - fx) * fy * cc + fx * fy * d + 32) >> 6;
+}
+
+// Rows r0, r0 + dr, ... < n of column c of a chroma block, K rows per lane at
+// a time, loads first; rows past the block re-read its last and are not stored.
+template <int K>
+__device__ __forceinline__ void chroma_rows(const PicDev &R, int xi, int yi, int fx, int fy,
+                                            gout8 o, int ostride, int c, int r0, int dr, int n)
+{
+    for (int rb = r0; rb < n; rb += K * dr) {
+        int v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = chroma_px(R, xi, yi, fx, fy, c, min(rb + k * dr, n - 1));
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (rb + k * dr < n) o[static_cast<int64_t>(rb + k * dr) * ostride + c] = static_cast<uint8_t>(v[k]);
+    }
+}
+
+// Chroma at 1/8 sample (tc_inter_predict_chroma, src/motion.c:597-632).  The
+// list is taken in quads of consecutive blocks (mc_share over quads): when all
+// four are at most 4x4 (a 4:2:0 8x8 luma node's chroma, most of a frame's
+// blocks), each takes 16 lanes of the wavefront, its descriptor loaded per
+// lane; otherwise the wavefront predicts them one after another, up to 4
+// rows per lane at a time.
 __global__ __launch_bounds__(256) void mc_chroma_kernel(const PicDev *__restrict__ table,
                                                         int npics,
                                                         const tcme_pred *__restrict__ preds,
@@ -287,28 +323,44 @@ __global__ __launch_bounds__(256) void mc_chroma_kernel(const PicDev *__restrict
 {
     const int lane = static_cast<int>(threadIdx.x & 63);
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-    const McShare sh = mc_share(n, wave);
+    const McShare sh = mc_share((n + 3) / 4, wave);
     PicDev R{};
     int cached = -1;
-    for (int64_t i = sh.first; i < sh.end; i += sh.step) {
-        const tcme_pred p = preds[i];
-        const int st = check_pred(table, npics, p, out_bytes, false, R, cached);
-        if (status && lane == 0) status[i] = st;
-        if (st) continue;
-        const int xi = p.mv_x >> 3, yi = p.mv_y >> 3, fx = p.mv_x & 7, fy = p.mv_y & 7;
-        const gptr8 Y = (gptr8)R.y;
-        const gout8 o = (gout8)out + p.out_off;
-        const BlockLanes bl(p.blk, lane);
-        const int c = bl.c;
-        const int x0 = min(max(xi + c, 0), R.w - 1), x1 = min(max(xi + c + 1, 0), R.w - 1);
-        for (int r = bl.r0; r < p.blk; r += bl.dr) {
-            const int y0 = min(max(yi + r, 0), R.h - 1), y1 = min(max(yi + r + 1, 0), R.h - 1);
-            const int a = Y[static_cast<size_t>(y0) * R.pitch + x0];
-            const int b = Y[static_cast<size_t>(y0) * R.pitch + x1];
-            const int cc = Y[static_cast<size_t>(y1) * R.pitch + x0];
-            const int d = Y[static_cast<size_t>(y1) * R.pitch + x1];
-            const int v = (8 - fx) * (8 - fy) * a + fx * (8 - fy) * b + (8 - fx) * fy * cc + fx * fy * d;
-            o[static_cast<int64_t>(r) * p.out_stride + c] = static_cast<uint8_t>((v + 32) >> 6);
+    for (int64_t q = sh.first; q < sh.end; q += sh.step) {
+        const int64_t i0 = q * 4;
+        const int cnt = n - i0 < 4 ? static_cast<int>(n - i0) : 4;
+        bool small = cnt == 4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (small) small = preds[i0 + k].blk >= 1 && preds[i0 + k].blk <= 4;
+        if (small) {
+            const int g = lane >> 4, l = lane & 15;
+            const tcme_pred p = preds[i0 + g];
+            PicDev Rg{};
+            int none = -1;
+            const int st = check_pred(table, npics, p, out_bytes, false, Rg, none);
+            if (status && l == 0) status[i0 + g] = st;
+            if (st == 0 && l < p.blk * p.blk) {
+                const int r = l / p.blk, c = l - r * p.blk;
+                const gout8 o = (gout8)out + p.out_off;
+                o[static_cast<int64_t>(r) * p.out_stride + c] = static_cast<uint8_t>(
+                    chroma_px(Rg, p.mv_x >> 3, p.mv_y >> 3, p.mv_x & 7, p.mv_y & 7, c, r));
+            }
+            continue;
+        }
+        for (int k = 0; k < cnt; ++k) {
+            const int64_t i = i0 + k;
+            const tcme_pred p = preds[i];
+            const int st = check_pred(table, npics, p, out_bytes, false, R, cached);
+            if (status && lane == 0) status[i] = st;
+            if (st) continue;
+            const int xi = p.mv_x >> 3, yi = p.mv_y >> 3, fx = p.mv_x & 7, fy = p.mv_y & 7;
+            const gout8 o = (gout8)out + p.out_off;
+            const BlockLanes bl(p.blk, lane);
+            if ((p.blk + bl.dr - 1) / bl.dr >= 4)  // rows per lane; wave-uniform
+                chroma_rows<4>(R, xi, yi, fx, fy, o, p.out_stride, bl.c, bl.r0, bl.dr, p.blk);
+            else
+                chroma_rows<1>(R, xi, yi, fx, fy, o, p.out_stride, bl.c, bl.r0, bl.dr, p.blk);
         }
     }
 }

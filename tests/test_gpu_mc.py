@@ -156,6 +156,32 @@ def test_gpu_chroma_mc_matches_oracle(engine, orc, shape):
     engine.free(cp)
 
 
+def test_gpu_chroma_mc_small_block_quads(engine, orc):
+    """Runs of 1x1..4x4 blocks (four consecutive ones share a wavefront)
+    between larger blocks, quads that mix the two, a rejected descriptor inside
+    a run of small blocks, and a list whose length is not a multiple of 4."""
+    print("chroma mc small-block quads", flush=True)
+    h, w = 40, 64
+    pl, _ = moving_pair(h, w, 9)
+    cp = engine.plane(pl)
+    rng = np.random.default_rng(31)
+    sizes = np.array([4] * 8 + [8] + [2, 3, 4, 1] + [16] + [4] * 5 + [1, 1, 3])
+    mvs = np.stack([rng.integers(-80, 8 * w + 80, len(sizes)),
+                    rng.integers(-80, 8 * h + 80, len(sizes))], 1)
+    preds = _preds(cp, mvs, sizes)
+    preds["pic"][2] = 999      # unknown picture, inside the first quad of 4x4 blocks
+    out, st = engine.predict(preds, chroma=True)
+    assert st[2] == -4 and (np.delete(st, 2) == 0).all()
+    for k in range(len(preds)):
+        if k == 2:
+            continue
+        n = int(sizes[k])
+        o = int(preds["out_off"][k])
+        want = orc.inter_predict_chroma(pl, int(mvs[k, 0]), int(mvs[k, 1]), n)
+        assert np.array_equal(out[o:o + n * n].reshape(n, n), want), (k, mvs[k], n)
+    engine.free(cp)
+
+
 def test_gpu_mc_strided_output_and_bad_descriptors(engine, orc):
     from tcodec_amd import me
     ref, _ = moving_pair(64, 96, 2)
