@@ -187,10 +187,23 @@ __global__ __launch_bounds__(kGmvThreads) void gmv_walk_kernel(const PicDev *__r
     }
     const int nx = (C.w / 4 + 1) / 2, ny = (C.h / 4 + 1) / 2, ns = nx * ny;
     const int nb = (ny + rows - 1) / rows;
-    for (int e = threadIdx.x; e < kGmvEntries; e += kGmvThreads) {
-        int64_t t = 0;
-        for (int b = 0; b < nb; ++b) t += part[(static_cast<size_t>(i) * nbands + b) * kGmvEntries + e];
-        tab[e] = t;
+    // the table is the sum of the bands' partial tables: bands outside, this
+    // thread's entries inside, so each band's loads are in flight together
+    // (entries outside made every load wait for the one before: 81 in a row)
+    constexpr int KE = (kGmvEntries + kGmvThreads - 1) / kGmvThreads;
+    int64_t acc[KE] = {};
+    for (int b = 0; b < nb; ++b) {
+        const int32_t *q = part + (static_cast<size_t>(i) * nbands + b) * kGmvEntries;
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+            const int e = static_cast<int>(threadIdx.x) + k * kGmvThreads;
+            if (e < kGmvEntries) acc[k] += q[e];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+        const int e = static_cast<int>(threadIdx.x) + k * kGmvThreads;
+        if (e < kGmvEntries) tab[e] = acc[k];
     }
     __syncthreads();
     int bx = 0, by = 0;
