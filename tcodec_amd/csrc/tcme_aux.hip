@@ -152,6 +152,7 @@ __global__ __launch_bounds__(256) void gmv_table_kernel(const PicDev *__restrict
         const uint32_t *rw = reinterpret_cast<const uint32_t *>(ref_s + (2 * (r4 - r4_base) + (ix & 1)) * hp);
         const uint32_t *cw = reinterpret_cast<const uint32_t *>(cr);
         uint32_t vacc = 0;
+#pragma unroll 8
         for (int i = ia; i < ib; i += 4) {
             const int b = (i + hofs) >> 2;
             vacc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(rw[b + 1], rw[b], sh), cw[i >> 2],
