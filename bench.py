@@ -257,6 +257,44 @@ def dropin_rate(frames, scene, pics, samples: int = 400):
     return samples / dt / len(calls)
 
 
+def host_api_rate(eng, frames, reps: int = 2):
+    """Every call of each timed frame through tcme_search_host: the call list
+    in host memory is copied to the device, searched in one launch, and the
+    results copied back before the call returns (PCIe included).  ME frames/s
+    and ms per 1080p frame."""
+    lists = [c for _, _, c in frames]
+    eng.search(lists[0])  # the context's staging grows outside the timing
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for c in lists:
+            eng.search(c)
+    dt = time.perf_counter() - t0
+    k = reps * len(lists)
+    out = {"frames_per_s": round(k / dt, 1), "ms_per_frame": round(dt / k * 1e3, 3),
+           "bytes_per_frame": int(len(lists[0]) * (32 + 16)),
+           "note": "tcme_search_host: calls in and results out through host memory"}
+    # the same with the caller's buffers in pinned host memory: direct DMA
+    import torch
+    from tcodec_amd import me
+    pin_calls = [torch.from_numpy(c.view(np.uint8).copy()).pin_memory() for c in lists]
+    pin_res = torch.empty(len(lists[0]) * 16, dtype=torch.uint8).pin_memory()
+    assert all(c.numel() // 32 <= pin_res.numel() // 16 for c in pin_calls)
+
+    def run(ci):
+        me._check(eng.L.tcme_search_host(eng.h, pin_calls[ci].data_ptr(), pin_calls[ci].numel() // 32,
+                                         pin_res.data_ptr(), None, me.C.c_void_p(0)))
+
+    run(0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for ci in range(len(lists)):
+            run(ci)
+    dt = time.perf_counter() - t0
+    out["pinned"] = {"frames_per_s": round(k / dt, 1), "ms_per_frame": round(dt / k * 1e3, 3)}
+    log(f"host_api: {out}")
+    return out
+
+
 def encoder_leg():
     """The reference encoder on the CPU-runnable configuration - 320x240, 8
     frames of the diagonal-gradient source (+3,-2) px per frame with noise,
@@ -629,6 +667,7 @@ def main():
         encoder = {"error": f"{type(e).__name__}: {e}"[:300]}
     log(f"encoder: {encoder}")
     dropin = None if args.no_forms else dropin_rate(frames, scene, pics)
+    host_api = host_api_rate(eng, frames)
     batched = None if args.no_forms else batched_form(eng, args.batch_steps)
     served = None if args.no_forms else served_form(eng, args.serve_rate, args.serve_seconds,
                                                     args.serve_budget_gib)
@@ -676,6 +715,7 @@ def main():
                              "us_per_call": round(1e6 / (dropin * calls_per_frame), 1),
                              "note": "tc_motion_est, one launch per call, host planes"}
                             if dropin else None),
+        "host_api": host_api,
         "mc": mc,
         "aux": aux,
         "encoder": encoder,
