@@ -142,8 +142,11 @@ typedef struct {
     orc_stats *st;
 } me_ctx;
 
-/* Integer candidate cost, src/motion.c:371-380: off-frame -> INT32_MAX. */
-static int32_t cost_int(const me_ctx *m, int x, int y)
+/* Integer candidate cost, src/motion.c:371-380: off-frame -> INT32_MAX.
+ * Coordinates are taken as 64-bit: the reference forms them in int
+ * (motion.c:401-402), which overflows for search ranges near INT_MAX; read
+ * mathematically such a candidate is simply off the frame. */
+static int32_t cost_int(const me_ctx *m, long long x, long long y)
 {
     if (x < 0 || y < 0 || x + m->n > m->w || y + m->n > m->h) {
         if (m->st) m->st->int_oob++;
@@ -188,9 +191,10 @@ static void stage_int(const me_ctx *m, const int8_t (*pat)[2], int npat, int sca
     for (int it = 0; it < passes; it++) {
         int moved = 0;
         for (int k = 0; k < npat; k++) {
-            int x = *bx + pat[k][0] * scale, y = *by + pat[k][1] * scale;
+            long long x = *bx + (long long)pat[k][0] * scale;
+            long long y = *by + (long long)pat[k][1] * scale;
             int32_t s = cost_int(m, x, y);
-            if (s < *best) { *best = s; *bx = x; *by = y; moved = 1; }
+            if (s < *best) { *best = s; *bx = (int)x; *by = (int)y; moved = 1; }
         }
         if (!moved) return;
     }

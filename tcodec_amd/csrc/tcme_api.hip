@@ -842,7 +842,10 @@ TCME_API tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, in
     // Reachable window: two large-hex passes (16*scale), three small-hex
     // passes (12), four diamond passes (4), the quarter-pel floor (1) and the
     // six-tap apron (2 left/up, 3 right/down) - margin 16*scale + 24.
-    const int scale = search_range / 4 > 1 ? search_range / 4 : 1;
+    // (the step is capped at the larger side, as the kernels do: beyond that
+    // every large-hex point is off the picture and the window is the whole
+    // picture anyway)
+    const int scale = tcme::search_scale(search_range, ref_w, ref_h);
     const int reach = 16 * scale + 24;
     int cx = center_x < 0 ? 0 : (center_x > ref_w - n ? ref_w - n : center_x);
     int cy = center_y < 0 ? 0 : (center_y > ref_h - n ? ref_h - n : center_y);

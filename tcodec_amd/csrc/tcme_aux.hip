@@ -282,14 +282,13 @@ __global__ __launch_bounds__(256) void sad_kernel(const PicDev *__restrict__ tab
         else {
             A = table[d.pic];
             if (!A.y) status = TCME_E_PIC;
-            else if (d.x < 0 || d.y < 0 || d.x + d.w > A.w || d.y + d.h > A.h) status = TCME_E_ARG;
+            else if (!rect_fits(d.x, d.y, d.w, d.h, A.w, A.h)) status = TCME_E_ARG;
             else if (d.other_pic >= 0) {
                 B = table[d.other_pic];
                 if (!B.y) status = TCME_E_PIC;
-                else if (d.ox < 0 || d.oy < 0 || d.ox + d.w > B.w || d.oy + d.h > B.h)
+                else if (!rect_fits(d.ox, d.oy, d.w, d.h, B.w, B.h))
                     status = TCME_E_ARG;
-            } else if (!buf || d.buf_stride < d.w || d.buf_off < 0 ||
-                       d.buf_off + static_cast<int64_t>(d.h - 1) * d.buf_stride + d.w > buf_bytes) {
+            } else if (!buf || !span_fits(d.buf_off, d.w, d.h, d.buf_stride, buf_bytes)) {
                 status = TCME_E_ARG;
             }
         }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "tcme.h"
+#include "tcme_bounds.h"
 
 namespace tcme {
 

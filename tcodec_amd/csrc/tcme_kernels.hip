@@ -111,7 +111,7 @@ struct Counts { int ie = 0, oob = 0, q6 = 0, qb = 0; };
 
 __device__ __forceinline__ bool fits(const PicDev &R, int n, int x, int y)
 {
-    return x >= 0 && y >= 0 && x + n <= R.w && y + n <= R.h;
+    return block_fits(x, y, n, R.w, R.h);
 }
 
 // ---- anchors ---------------------------------------------------------------
@@ -653,7 +653,7 @@ __device__ void search(const PicDev &R, const PicDev &C, const tcme_call &c, tcm
     else { best = kInf; S.cnt.oob++; }
     int qx, qy;
     if (R.w >= N && R.h >= N) {
-        const int scale = c.sr / 4 > 1 ? c.sr / 4 : 1;
+        const int scale = search_scale(c.sr, R.w, R.h);
         S.template stage<PAT_LARGE>(scale, 2, bx, by, best);
         S.template stage<PAT_SMALL>(1, 3, bx, by, best);
         S.template stage<PAT_DIAMOND>(1, 4, bx, by, best);
@@ -698,7 +698,7 @@ __device__ __forceinline__ int check_call(const PicDev *table, int npics, const 
     R = table[c.ref_pic];
     C = table[c.cur_pic];
     if (!R.y || !R.an || !C.y) return TCME_E_PIC;  // empty slot, or a reference without anchors
-    if (c.px < 0 || c.py < 0 || c.px + c.blk > C.w || c.py + c.blk > C.h) return TCME_E_ARG;
+    if (!block_fits(c.px, c.py, c.blk, C.w, C.h)) return TCME_E_ARG;
     return 0;
 }
 
@@ -1071,7 +1071,7 @@ __device__ __forceinline__ void run_slot(TileShared<WAVES> &sh, const PicDev *__
         else { ++oob; }
         int qx = bx * 4, qy = by * 4, q6 = 0;
         if (R.w >= N && R.h >= N) {
-            const int scale = c.sr / 4 > 1 ? c.sr / 4 : 1;
+            const int scale = search_scale(c.sr, R.w, R.h);
             G.template stage<PAT_LARGE>(scale, 2, bx, by, best, ie, oob);
             G.template stage<PAT_SMALL>(1, 3, bx, by, best, ie, oob);
             G.template stage<PAT_DIAMOND>(1, 4, bx, by, best, ie, oob);
@@ -1228,15 +1228,15 @@ __global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile
     int pos = 0;
     while (pos < nlive) {
         const int ref = sh.call[pos].ref_pic;
-        const int tx = sh.call[pos].cx >> 6, ty = sh.call[pos].cy >> 6;
+        const int tx = centre_tile(sh.call[pos].cx), ty = centre_tile(sh.call[pos].cy);
         // the run ends at the first later call with another reference or tile
         int end = nlive;
         for (int p0 = pos + 1; p0 < nlive; p0 += 64) {
             const int p = p0 + lane;
             bool brk = false;
             if (p < nlive) {
-                brk = sh.call[p].ref_pic != ref || (sh.call[p].cx >> 6) != tx ||
-                      (sh.call[p].cy >> 6) != ty;
+                brk = sh.call[p].ref_pic != ref || centre_tile(sh.call[p].cx) != tx ||
+                      centre_tile(sh.call[p].cy) != ty;
             }
             const uint64_t m = __ballot(brk);
             if (m) { end = p0 + __builtin_ctzll(m); break; }

@@ -102,9 +102,7 @@ __device__ __forceinline__ int check_pred(const PicDev *table, int npics, const 
     }
     if (!R.y || (need_anchors && !R.an)) return TCME_E_PIC;
     if (p.blk < 1 || p.blk > kMaxBlk) return TCME_E_BLK;
-    if (p.out_stride < p.blk || p.out_off < 0 ||
-        p.out_off + static_cast<int64_t>(p.blk - 1) * p.out_stride + p.blk > out_bytes)
-        return TCME_E_ARG;
+    if (!span_fits(p.out_off, p.blk, p.blk, p.out_stride, out_bytes)) return TCME_E_ARG;
     return 0;
 }
 

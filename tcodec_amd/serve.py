@@ -189,16 +189,26 @@ class Server:
         self.max_tick_calls = max_tick_calls
         self.keep = keep_results
         self.tpl = Templates()
-        self.stream = torch.cuda.current_stream()
-        self.sptr = self.stream.cuda_stream
+        self._stream = None
         self.queue: list[Stream] = []
         self.active: list[Stream] = []
         self.finished: list[Stream] = []
+        self.dropped: list[Stream] = []   # requests larger than the whole budget
         self.used = 0
         self.frames = 0           # searched (non-intra) pictures
         self.calls = 0
         self.ticks = 0
         self.tick_ms: list[float] = []
+
+    @property
+    def stream(self):
+        if self._stream is None:
+            self._stream = torch.cuda.current_stream()
+        return self._stream
+
+    @property
+    def sptr(self) -> int:
+        return self.stream.cuda_stream
 
     def submit(self, s: Stream) -> None:
         self.queue.append(s)
@@ -219,6 +229,12 @@ class Server:
             s = self.queue[0]
             need = stream_bytes(s.w, s.h)
             cost = self.calls_per_frame(s)
+            if need > self.budget:
+                # could never be admitted, not even by an idle server: refuse
+                # it rather than let it block every request behind it
+                self.queue.pop(0)
+                self.dropped.append(s)
+                continue
             if self.used + need > self.budget or (self.active and work + cost > self.max_tick_calls):
                 break
             self.queue.pop(0)
@@ -289,6 +305,8 @@ class Server:
             now = time.perf_counter()
             self._admit(now)
             if not self.active:
+                if not self.queue:
+                    break
                 wait = self.queue[0].arrival - now
                 if wait > 0:
                     time.sleep(min(wait, 0.05))

@@ -148,6 +148,43 @@ def test_gpu_rejects_bad_descriptors(engine):
     engine.free(rp)
 
 
+def test_gpu_int32_edge_descriptors(engine, orc):
+    """Descriptors at the int32 edges: block origins near INT_MAX are rejected
+    (their x + n used to wrap), and search ranges near INT_MAX give the
+    oracle's answer (every large-hexagon point off the picture) in every
+    launch shape and through the drop-in symbol."""
+    from tcodec_amd import me
+    log("int32 edges")
+    big = 2**31 - 1
+    ref, cur = moving_pair(96, 128, 5, shift=(2, -3))
+    rp, cp = engine.pic(ref), engine.pic(cur)
+    try:
+        bad = me.make_calls(rp, cp, [big - 3, 0, big - 63, 0], [0, big - 3, 0, big], 0, 0,
+                            [8, 8, 64, 16], 24)
+        res = engine.search(bad)
+        assert (res["status"] == -1).all(), res["status"]
+        rng = np.random.default_rng(9)
+        rows = random_calls(rng, 128, 96, 200, srs=(big, big - 3, 2**30, 4 * 128 + 7))
+        rows[::7, 2] = big                       # raw centres at the edges too
+        rows[3::7, 3] = -big - 1
+        calls = me.make_calls(rp, cp, rows[:, 0], rows[:, 1], rows[:, 2], rows[:, 3],
+                              rows[:, 4], rows[:, 5])
+        want = orc.motion_est_batch(ref, cur, rows)
+        t0 = engine.tiling()
+        for t in ((1, 1, 1, 4, 8, 85, 1), (1, 1, 2, 4, 4, 85, 0), (1, 1, 1, 2, 0, 85, 0)):
+            engine.set_tiling(*t)
+            compare(engine.search(calls), want, calls, f"huge sr {t}")
+        engine.set_tiling(*[getattr(t0, k) for k, _ in me.Tiling._fields_])
+        for k in range(0, len(rows), 10):
+            px, py, cx, cy, n, sr = (int(v) for v in rows[k])
+            (mx, my), sad = me.motion_est(ref, 128, 128, 96, cur, py * 128 + px, 128, cx, cy,
+                                          n, sr)
+            assert (mx, my, sad) == tuple(want[k]), rows[k]
+    finally:
+        engine.free(rp)
+        engine.free(cp)
+
+
 def test_gpu_full_hd_sample(engine, orc):
     """A sample of the 1080p B workload checked exactly (the oracle is too
     slow for every call), plus the whole frame's call set run for status and

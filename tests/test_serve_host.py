@@ -1,0 +1,26 @@
+"""Host logic of the serving loop (no GPU): admission by HBM budget."""
+import time
+
+from tcodec_amd import serve
+
+
+def _stream(sid, w, h, arrival):
+    return serve.Stream(sid, w, h, 5, "P", clip=None, arrival=arrival)
+
+
+def test_request_larger_than_budget_is_refused_not_blocking():
+    """A request whose pictures exceed the whole budget is refused (recorded
+    in `dropped`) instead of blocking the queue forever, and run() returns."""
+    srv = serve.Server(eng=None, hbm_budget=serve.stream_bytes(320, 240) - 1)
+    srv.submit(_stream(0, 3840, 2160, 0.0))
+    srv.submit(_stream(1, 1920, 1080, 0.01))
+    t0 = time.perf_counter()
+    srv.run()
+    assert time.perf_counter() - t0 < 5
+    assert [s.sid for s in srv.dropped] == [0, 1]
+    assert not srv.active and not srv.queue and not srv.finished and srv.used == 0
+
+
+def test_stream_bytes_counts_five_resident_pictures():
+    assert serve.stream_bytes(1920, 1080) == 5 * serve.pic_bytes(1920, 1080)
+    assert serve.pic_bytes(320, 240) == 320 * 240 * 9 + 512
