@@ -16,10 +16,19 @@
  * At exit one line goes to stderr:
  *   TCME_SHIM mode=<m> calls=<n> mismatches=<k> me_seconds=<t> first_call_seconds=<f>
  * (me_seconds includes the first call, which in gpu mode also starts the GPU
- * runtime).
+ * runtime), then one line per call site (the caller's return address; a site
+ * the compiler duplicated by inlining counts once per copy):
+ *   TCME_SITE ra=<offset> calls=<n> slot1=<a> glob_centre=<g> bframe=<b>
+ * slot1: calls against dpb[1] (the multi-reference search, encoder.c:982);
+ * glob_centre: calls centred on the global-motion centre of a P frame with
+ * |glob| > 16 quarter samples (encoder.c:1000-1004); bframe: calls against a
+ * picture that is not the most recent reference (B forward/backward,
+ * encoder.c:807, 834, or dpb[1..3]).  Counters are atomic: the v1 encoder
+ * calls from its WPP pool threads (encoder.c:3014).
  */
 #include "tcodec_common.h"
 #include <dlfcn.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -39,6 +48,11 @@ static int g_mode = -1;
 static me_fn_t g_gpu_fn;
 static long g_calls, g_mism;
 static double g_sec, g_first;
+static pthread_once_t g_once = PTHREAD_ONCE_INIT;
+static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;  /* sites, timing, trace */
+#define MAX_SITES 32
+static struct { void *ra; long calls, slot1, glob, bframe; } g_site[MAX_SITES];
+static int g_nsites;
 static tc_encoder_t *g_enc;
 static FILE *g_trace;
 #define MAX_SEEN 4096
@@ -63,7 +77,45 @@ static void shim_report(void)
 {
     fprintf(stderr, "TCME_SHIM mode=%s calls=%ld mismatches=%ld me_seconds=%.6f "
             "first_call_seconds=%.6f\n", mode_name(), g_calls, g_mism, g_sec, g_first);
+    Dl_info di;
+    for (int i = 0; i < g_nsites; i++) {
+        long off = (dladdr(g_site[i].ra, &di) && di.dli_fbase)
+                       ? (long)((const char *)g_site[i].ra - (const char *)di.dli_fbase) : 0;
+        fprintf(stderr, "TCME_SITE ra=0x%lx calls=%ld slot1=%ld glob_centre=%ld bframe=%ld\n",
+                off, g_site[i].calls, g_site[i].slot1, g_site[i].glob, g_site[i].bframe);
+    }
     if (g_trace) fclose(g_trace);
+}
+
+/* Books a call against the site that made it (see the header comment). */
+static void count_site(void *ra, const tc_pixel_t *ref, const tc_pixel_t *orig, int cx, int cy)
+{
+    int slot = -1, glob = 0;
+    if (g_enc) {
+        for (int i = 0; i < TC_REF_FRAMES; i++)
+            if (g_enc->dpb[i].frame && g_enc->dpb[i].frame->y == ref) slot = i;
+        const tc_frame_buf_t *cur = g_enc->cur;
+        if (slot == 0 && cur && g_enc->glob_mv_valid &&
+            (g_enc->glob_mv_x > 16 || g_enc->glob_mv_x < -16 ||
+             g_enc->glob_mv_y > 16 || g_enc->glob_mv_y < -16) &&
+            orig >= cur->y && orig < cur->y + (size_t)cur->stride_y * cur->height) {
+            size_t off = (size_t)(orig - cur->y);
+            int px = (int)(off % (size_t)cur->stride_y), py = (int)(off / (size_t)cur->stride_y);
+            glob = cx == ((px * 4 + g_enc->glob_mv_x) >> 2) &&
+                   cy == ((py * 4 + g_enc->glob_mv_y) >> 2);
+        }
+    }
+    pthread_mutex_lock(&g_mu);
+    int k = 0;
+    while (k < g_nsites && g_site[k].ra != ra) k++;
+    if (k == g_nsites && g_nsites < MAX_SITES) g_site[g_nsites++].ra = ra;
+    if (k < MAX_SITES) {
+        g_site[k].calls++;
+        g_site[k].slot1 += slot == 1;
+        g_site[k].glob += glob;
+        g_site[k].bframe += slot != 0;
+    }
+    pthread_mutex_unlock(&g_mu);
 }
 
 static void shim_init(void)
@@ -157,37 +209,39 @@ tc_mv_s tc_motion_est(const tc_pixel_t *ref, int ref_stride, int ref_w, int ref_
                       int center_y, int blk_size, int search_range,
                       tc_sad_t *best_sad_out)
 {
-    if (g_mode < 0) shim_init();
-    g_calls++;
+    pthread_once(&g_once, shim_init);
+    const long call = __atomic_add_fetch(&g_calls, 1, __ATOMIC_RELAXED);
+    count_site(__builtin_return_address(0), ref, orig, center_x, center_y);
     tc_sad_t sad = 0;
     tc_mv_s mv;
+    double t0 = now_s();
     if (g_mode == MODE_GPU) {
-        double t0 = now_s();
         mv = g_gpu_fn(ref, ref_stride, ref_w, ref_h, orig, orig_stride, center_x, center_y,
                       blk_size, search_range, &sad);
-        g_sec += now_s() - t0;
-        if (g_calls == 1) g_first = g_sec;
     } else {
-        double t0 = now_s();
         mv = tc_motion_est_ref(ref, ref_stride, ref_w, ref_h, orig, orig_stride, center_x,
                                center_y, blk_size, search_range, &sad);
-        g_sec += now_s() - t0;
-        if (g_calls == 1) g_first = g_sec;
-        if (g_mode == MODE_CHECK) {
-            tc_sad_t gsad = 0;
-            tc_mv_s gmv = g_gpu_fn(ref, ref_stride, ref_w, ref_h, orig, orig_stride,
-                                   center_x, center_y, blk_size, search_range, &gsad);
-            if (gmv.x != mv.x || gmv.y != mv.y || gsad != sad) {
-                if (g_mism < 10)
-                    fprintf(stderr, "TCME_SHIM mismatch call=%ld c=(%d,%d) n=%d sr=%d "
-                            "ref=(%d,%d,%d) gpu=(%d,%d,%d)\n", g_calls, center_x, center_y,
-                            blk_size, search_range, mv.x, mv.y, sad, gmv.x, gmv.y, gsad);
-                g_mism++;
-            }
-        } else if (g_mode == MODE_TRACE) {
-            trace_call(ref, ref_stride, ref_w, ref_h, orig, orig_stride, center_x, center_y,
-                       blk_size, search_range, mv, sad);
+    }
+    const double dt = now_s() - t0;
+    pthread_mutex_lock(&g_mu);
+    g_sec += dt;
+    if (call == 1) g_first = dt;
+    pthread_mutex_unlock(&g_mu);
+    if (g_mode == MODE_CHECK) {
+        tc_sad_t gsad = 0;
+        tc_mv_s gmv = g_gpu_fn(ref, ref_stride, ref_w, ref_h, orig, orig_stride,
+                               center_x, center_y, blk_size, search_range, &gsad);
+        if (gmv.x != mv.x || gmv.y != mv.y || gsad != sad) {
+            if (__atomic_add_fetch(&g_mism, 1, __ATOMIC_RELAXED) <= 10)
+                fprintf(stderr, "TCME_SHIM mismatch call=%ld c=(%d,%d) n=%d sr=%d "
+                        "ref=(%d,%d,%d) gpu=(%d,%d,%d)\n", call, center_x, center_y,
+                        blk_size, search_range, mv.x, mv.y, sad, gmv.x, gmv.y, gsad);
         }
+    } else if (g_mode == MODE_TRACE) {
+        pthread_mutex_lock(&g_mu);
+        trace_call(ref, ref_stride, ref_w, ref_h, orig, orig_stride, center_x, center_y,
+                   blk_size, search_range, mv, sad);
+        pthread_mutex_unlock(&g_mu);
     }
     if (best_sad_out) *best_sad_out = sad;
     return mv;

@@ -85,6 +85,90 @@ class DeviceGradientClip(Clip):
         self.host = None
 
 
+class DeviceSceneClip(Clip):
+    """The moving-rectangle scene of ``synth.Scene`` (smooth value-noise
+    background scaled to 16..235, K textured rectangles 32..256 px per side
+    moving at quarter-pel velocities in [-16, 16] px/frame and wrapping at the
+    frame edge, a global pan, Gaussian noise sigma 2) rendered on the device
+    in float32 with the device's own RNG, so that long clips of large pictures
+    (every picture distinct) cost no host time.  Same structure as the host
+    generator; the values differ from it."""
+
+    PERIODS, AMPS = (128, 48, 16), (1.0, 0.5, 0.25)
+
+    def __init__(self, w: int, h: int, frames: int, seed: int, rects: int | None = None,
+                 pan: tuple[float, float] = (0.75, 0.25), sigma: float = 2.0,
+                 device: str = "cuda"):
+        self.w, self.h, self.frames = w, h, frames
+        self.host = None
+        dev = torch.device(device)
+        rects = (48 if w >= 1920 else 16) if rects is None else rects
+        rng = np.random.default_rng(seed)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        rw = rng.integers(32, 257, rects)
+        rh = rng.integers(32, 257, rects)
+        x0 = rng.random(rects) * (w + rw) - rw
+        y0 = rng.random(rects) * (h + rh) - rh
+        vx = rng.integers(-64, 65, rects) / 4.0
+        vy = rng.integers(-64, 65, rects) / 4.0
+        base = rng.integers(40, 216, rects).astype(np.float64)
+        tex = [(torch.rand((int(rh[k]) // 8 + 2, int(rw[k]) // 8 + 2), generator=gen,
+                           device=dev) * 2 - 1) * 40.0 for k in range(rects)]
+        # one lattice per octave covering every frame's panned window
+        ox = [-pan[0] * t for t in (0, frames - 1)]
+        oy = [-pan[1] * t for t in (0, frames - 1)]
+        lats = []
+        for per in self.PERIODS:
+            gx0 = int(np.floor(min(ox) / per)) - 1
+            gy0 = int(np.floor(min(oy) / per)) - 1
+            gx1 = int(np.floor((max(ox) + w) / per)) + 2
+            gy1 = int(np.floor((max(oy) + h) / per)) + 2
+            lats.append((gx0, gy0, torch.rand((gy1 - gy0 + 1, gx1 - gx0 + 1), generator=gen,
+                                              device=dev)))
+        xs = torch.arange(w, device=dev, dtype=torch.float32)
+        ys = torch.arange(h, device=dev, dtype=torch.float32)
+        out = torch.empty((frames, h, w), dtype=torch.uint8, device=dev)
+        for t in range(frames):
+            acc = torch.zeros((h, w), device=dev)
+            for per, amp, (gx0, gy0, lat) in zip(self.PERIODS, self.AMPS, lats):
+                u = (xs - pan[0] * t) / per
+                v = (ys - pan[1] * t) / per
+                iu, iv = torch.floor(u), torch.floor(v)
+                fu, fv = u - iu, v - iv
+                fu = fu * fu * (3 - 2 * fu)
+                fv = fv * fv * (3 - 2 * fv)
+                cu = (iu.long() - gx0)
+                cv = (iv.long() - gy0)
+                l0, l1 = lat[cv], lat[cv + 1]
+                top = l0[:, cu] + (l0[:, cu + 1] - l0[:, cu]) * fu[None, :]
+                bot = l1[:, cu] + (l1[:, cu + 1] - l1[:, cu]) * fu[None, :]
+                acc += amp * (top + (bot - top) * fv[:, None])
+            img = 16.0 + acc / sum(self.AMPS) * (235.0 - 16.0)
+            for k in range(rects):
+                kw, kh = int(rw[k]), int(rh[k])
+                px = (x0[k] + vx[k] * t + kw) % (w + kw) - kw
+                py = (y0[k] + vy[k] * t + kh) % (h + kh) - kh
+                xa, xb = max(0, int(np.ceil(px))), min(w, int(np.ceil(px + kw)))
+                ya, yb = max(0, int(np.ceil(py))), min(h, int(np.ceil(py + kh)))
+                if xa >= xb or ya >= yb:
+                    continue
+                T = tex[k]
+                u = (xs[xa:xb] - float(px)) / 8.0
+                v = (ys[ya:yb] - float(py)) / 8.0
+                iu = torch.clamp(torch.floor(u).long(), max=T.shape[1] - 2)
+                iv = torch.clamp(torch.floor(v).long(), max=T.shape[0] - 2)
+                fu = (u - iu)[None, :]
+                fv = (v - iv)[:, None]
+                t0, t1 = T[iv], T[iv + 1]
+                top = t0[:, iu] + (t0[:, iu + 1] - t0[:, iu]) * fu
+                bot = t1[:, iu] + (t1[:, iu + 1] - t1[:, iu]) * fu
+                img[ya:yb, xa:xb] = float(base[k]) + top + (bot - top) * fv
+            nz = torch.randn((h, w), generator=gen, device=dev) * sigma
+            out[t] = torch.round(img + nz).clamp_(0, 255).to(torch.uint8)
+        self.dev = out
+
+
 def frame_template(w: int, h: int, refs: int, sr: int) -> np.ndarray:
     """A picture's call list with picture ids 0..refs-1 for the references
     and `refs` for the current picture: every quadtree node against every
@@ -124,7 +208,10 @@ class Stream:
     slot_of: dict = field(default_factory=dict)  # poc -> ring index
     ready_at: float = 0.0
     latencies: list = field(default_factory=list)
-    results: list = field(default_factory=list)
+    results: list = field(default_factory=list)   # keep_results=True: every picture's results
+    digests: list = field(default_factory=list)   # keep_results="digest": one per picture
+    keep_at: set = field(default_factory=set)     # ... and these coded positions in full
+    kept: dict = field(default_factory=dict)
 
     def __post_init__(self):
         if self.gop == "B":
@@ -146,13 +233,18 @@ class Stream:
             eng.free(p)
         self.ring = []
 
-    def prepare(self, eng: me.Engine, tpl: Templates, stream_ptr: int, trace=None):
-        """Loads the next coded picture into the ring and returns its call
-        list on the device (None for an intra picture).  Advances pos.  With a
-        trace.TraceWriter, the picture is recorded as a plane (its device
-        picture id is the plane id)."""
+    def prepare(self, eng: me.Engine, stream_ptr: int, trace=None):
+        """Loads the next coded picture into the ring (a device copy, anchors
+        rebuilt, on stream_ptr) and returns its search as (template key,
+        picture ids of the references then of the current picture), or None
+        for an intra picture.  Advances pos.  With a trace.TraceWriter, the
+        picture is recorded as a plane (its device picture id is the plane
+        id)."""
         poc, kind, dpb = self.order[self.pos]
         self.pos += 1
+        src = (self.offset + poc) % self.clip.frames
+        assert all((self.offset + p) % self.clip.frames != src for p in dpb), \
+            "a reference would be the same source picture as the current one"
         keep = set(dpb)
         slot = next(i for i in range(5) if i not in
                     {s for p, s in self.slot_of.items() if p in keep})
@@ -161,20 +253,53 @@ class Stream:
         eng.upload_device(self.ring[slot], self.clip.plane_ptr(self.offset + poc), self.w,
                           stream_ptr)
         if trace is not None:
-            t = (self.offset + poc) % self.clip.frames
-            host = self.clip.host[t] if self.clip.host is not None else \
-                self.clip.dev[t].cpu().numpy()
+            host = self.clip.host[src] if self.clip.host is not None else \
+                self.clip.dev[src].cpu().numpy()
             trace.plane(self.ring[slot], host)
         if kind == "I" or not dpb:
             return None
         sr = workload.SR_B if kind == "B" else workload.SR_P
-        t = tpl.get(self.w, self.h, len(dpb), sr)
-        lut = torch.tensor([self.ring[self.slot_of[p]] for p in dpb] + [self.ring[slot]],
-                           dtype=torch.int32).cuda(non_blocking=True)
-        calls = t.clone()
-        calls[:, 0] = lut[t[:, 0].long()]
-        calls[:, 1] = lut[len(dpb)]
-        return calls
+        return (self.w, self.h, len(dpb), sr), [self.ring[self.slot_of[p]] for p in dpb] + \
+            [self.ring[slot]]
+
+
+def expand_calls(tpl: Templates, items) -> tuple[torch.Tensor, list[int]]:
+    """The call lists of a tick's pictures, concatenated in item order, built
+    on the device with one gather per template: items are (template key,
+    picture-id table) as Stream.prepare returns them.  Returns the int32
+    [N, 8] list and each picture's call count."""
+    groups: dict = {}
+    for i, (key, lut) in enumerate(items):
+        groups.setdefault(key, []).append(i)
+    parts: list = [None] * len(items)
+    for key, idx in groups.items():
+        t = tpl.get(*key)
+        luts = torch.tensor([items[i][1] for i in idx], dtype=torch.int32).cuda(non_blocking=True)
+        k, n = len(idx), t.shape[0]
+        calls = t.unsqueeze(0).expand(k, n, 8).clone()
+        calls[:, :, 0] = torch.gather(luts, 1, t[:, 0].long().unsqueeze(0).expand(k, n))
+        calls[:, :, 1] = luts[:, key[2]].unsqueeze(1)
+        for j, i in enumerate(idx):
+            parts[i] = calls[j]
+    sizes = [p.shape[0] for p in parts]
+    return (torch.cat(parts) if len(parts) > 1 else parts[0]), sizes
+
+
+def digests(res: torch.Tensor, sizes: list[int]) -> torch.Tensor:
+    """One int64 digest per picture of a tick's results (int32 [N, 4], the
+    pictures' calls concatenated): a sum of every field weighted by the
+    call's position inside its picture, on the device (wrapping int64).
+    Equal digests of the same picture served two ways are what the
+    alone-versus-mix checks compare when keeping every result would not fit."""
+    dev = res.device
+    r = res.long()
+    v = r[:, 0] * 1000003 + r[:, 1] * 999983 + r[:, 2] * 97 + r[:, 3]
+    sz = torch.tensor(sizes, dtype=torch.int64, device=dev)
+    starts = torch.cumsum(sz, 0) - sz
+    local = torch.arange(r.shape[0], device=dev) - torch.repeat_interleave(starts, sz)
+    cs = torch.cumsum(v * ((local * 2654435761) % 2147483647 + 1), 0)
+    ends = cs[torch.cumsum(sz, 0) - 1]
+    return ends - torch.cat([ends.new_zeros(1), ends[:-1]])
 
 
 class Server:
@@ -187,7 +312,7 @@ class Server:
         self.per_stream = per_stream_launch
         self.budget = hbm_budget
         self.max_tick_calls = max_tick_calls
-        self.keep = keep_results
+        self.keep = keep_results  # False, True (every result to the host) or "digest"
         self.tpl = Templates()
         self._stream = None
         self.queue: list[Stream] = []
@@ -199,6 +324,8 @@ class Server:
         self.calls = 0
         self.ticks = 0
         self.tick_ms: list[float] = []
+        self.max_active = 0           # most streams searched by one launch
+        self.admitted_while_busy = 0  # admissions joining streams already running
 
     @property
     def stream(self):
@@ -238,47 +365,59 @@ class Server:
             if self.used + need > self.budget or (self.active and work + cost > self.max_tick_calls):
                 break
             self.queue.pop(0)
+            if self.active:
+                self.admitted_while_busy += 1
             s.activate(self.eng)
             s.ready_at = now
             self.used += need
             work += cost
             self.active.append(s)
+            self.max_active = max(self.max_active, len(self.active))
 
     def tick(self, now: float) -> None:
         """One launch over the next picture of every active stream."""
-        lists, owners = [], []
+        items, owners = [], []
         for s in self.active:
-            c = s.prepare(self.eng, self.tpl, self.sptr, self.trace)
+            pos = s.pos
+            c = s.prepare(self.eng, self.sptr, self.trace)
             if c is not None:
-                lists.append(c)
-                owners.append((s, c.shape[0]))
-        if lists and self.per_stream:
-            # the plain form: one launch per picture, back to back
-            outs = []
-            for c in lists:
-                r = torch.empty((c.shape[0], 4), dtype=torch.int32, device="cuda")
-                self.eng.search_device(c.data_ptr(), c.shape[0], r.data_ptr(), None, self.sptr)
-                outs.append(r)
-            res = torch.cat(outs) if (self.keep or self.trace is not None) else None
-        elif lists:
-            calls = torch.cat(lists) if len(lists) > 1 else lists[0]
+                items.append(c)
+                owners.append((s, pos))
+        res = None
+        if items:
+            calls, sizes = expand_calls(self.tpl, items)
             res = torch.empty((calls.shape[0], 4), dtype=torch.int32, device="cuda")
-            self.eng.search_device(calls.data_ptr(), calls.shape[0], res.data_ptr(), None,
-                                   self.sptr)
+            if self.per_stream:
+                # the plain form: one launch per picture, back to back
+                o = 0
+                for n in sizes:
+                    self.eng.search_device(calls[o].data_ptr(), n, res[o].data_ptr(), None,
+                                           self.sptr)
+                    o += n
+            else:
+                self.eng.search_device(calls.data_ptr(), calls.shape[0], res.data_ptr(), None,
+                                       self.sptr)
+            dig = digests(res, sizes) if self.keep == "digest" else None
         self.stream.synchronize()
         end = time.perf_counter()
-        if lists and self.trace is not None:
+        if items and self.trace is not None:
             from .trace import rows_from
-            self.trace.calls(rows_from(torch.cat(lists).cpu().numpy(), res.cpu().numpy()))
-        if lists and self.keep:
-            host = res.cpu().numpy()
+            self.trace.calls(rows_from(calls.cpu().numpy(), res.cpu().numpy()))
+        if items and self.keep:
+            host = res.cpu().numpy() if self.keep is True else None
+            dh = dig.cpu().numpy() if dig is not None else None
             o = 0
-            for s, n in owners:
-                s.results.append(host[o:o + n].copy())
+            for k, ((s, pos), n) in enumerate(zip(owners, sizes)):
+                if host is not None:
+                    s.results.append(host[o:o + n].copy())
+                else:
+                    s.digests.append(int(dh[k]))
+                    if pos in s.keep_at:
+                        s.kept[pos] = res[o:o + n].cpu().numpy()
                 o += n
         self.ticks += 1
         self.tick_ms.append((end - now) * 1e3)
-        searched = {id(s): n for s, n in owners}
+        searched = {id(s): n for (s, _), n in zip(owners, sizes)} if items else {}
         still = []
         for s in self.active:
             if id(s) in searched:
@@ -335,16 +474,23 @@ def poisson_requests(rate: float, duration: float, seed: int, clips: dict,
         name = names[int(rng.choice(len(names), p=probs / probs.sum()))]
         w, h = RES[name]
         nf = int(rng.integers(frames[0], frames[1] + 1))
-        out.append(Stream(sid, w, h, nf, gop, clips[name], offset=int(rng.integers(0, 1000)),
-                          arrival=t))
+        # the stream's pictures are consecutive clip pictures when the clip
+        # is long enough (else it wraps round, which is a scene cut)
+        span = 4 * max(1, (nf - 1 + 3) // 4) + 1 if gop == "B" else nf
+        room = clips[name].frames - span
+        off = int(rng.integers(0, room + 1)) if room >= 0 else int(rng.integers(0, 1000))
+        out.append(Stream(sid, w, h, nf, gop, clips[name], offset=off, arrival=t))
         sid += 1
     return out
 
 
-def run_alone(eng: me.Engine, s: Stream) -> list[np.ndarray]:
-    """The same stream served by itself: its per-frame results."""
-    solo = Stream(s.sid, s.w, s.h, s.nframes, s.gop, s.clip, s.offset)
-    srv = Server(eng, hbm_budget=1 << 62, keep_results=True)
+def run_alone(eng: me.Engine, s: Stream, keep="results"):
+    """The same stream served by itself: its per-frame results (keep
+    "results"), or its per-picture digests and the pictures of s.keep_at in
+    full (keep "digest")."""
+    solo = Stream(s.sid, s.w, s.h, s.nframes, s.gop, s.clip, s.offset,
+                  keep_at=set(s.keep_at))
+    srv = Server(eng, hbm_budget=1 << 62, keep_results=True if keep == "results" else "digest")
     srv.submit(solo)
     srv.run()
-    return solo.results
+    return solo.results if keep == "results" else (solo.digests, solo.kept)

@@ -6,7 +6,12 @@ Binaries (built by oracle/Makefile from the reference sources):
   oracle/_ref/tcenc       reference encoder, CPU motion estimation
   oracle/_ref/tcenc_gpu   the same objects, tc_motion_est from libtcme.so
   oracle/_ref/tcenc_hook  the same objects, ME calls routed by oracle/me_shim.c
-  oracle/_ref/tcdec       reference decoder
+
+Each case runs the three encoders at once (separate processes): the CPU
+encoder, the drop-in encoder (bytes compared), and the check-mode encoder,
+which runs every call on both the reference and the GPU and compares them.
+The check mode also reports calls per call site, so a case can require that
+a site fired (multi-reference, global-motion centre, B forward/backward).
 """
 import hashlib
 import os
@@ -22,18 +27,34 @@ pytestmark = pytest.mark.gpu
 REF = os.path.join(ROOT, "oracle", "_ref")
 LIB = os.path.join(ROOT, "tcodec_amd", "lib", "libtcme.so")
 
+# name: (generator, w, h, frames, generator options, encoder flags, sites that must fire)
 CASES = {
-    # name: (generator, w, h, frames, flags)
-    "gradient_320x240_p_q32": ("gradient", 320, 240, 8, ["--v2", "-q", "32"]),
-    "scene_640x360_p_multiref_q27": ("scene", 640, 360, 4,
-                                     ["--v2", "--profile", "1", "-p", "2", "-q", "27"]),
-    "scene_480x272_bgop4_q32": ("scene", 480, 272, 9,
-                                ["--v2", "--profile", "1", "-p", "2", "-q", "32", "-b"]),
-    # the 720p and 1080p configurations' sizes and flags, on short clips
-    "scene_1280x720_p_multiref_q27": ("scene", 1280, 720, 4,
-                                      ["--v2", "--profile", "1", "-p", "2", "-q", "27"]),
-    "scene_1920x1080_bgop4_q32": ("scene", 1920, 1080, 5,
-                                  ["--v2", "--profile", "1", "-p", "2", "-q", "32", "-b"]),
+    # the CPU-runnable configuration: gradient 320x240, 8 frames, v2 P, 1 ref, QP 32
+    "config1_gradient_320x240_p_q32": ("gradient", 320, 240, 8, {}, ["--v2", "-q", "32"], ()),
+    "scene_640x360_p_multiref_q27": ("scene", 640, 360, 4, {"rects": 8, "pan": (0.75, 0.25)},
+                                     ["--v2", "--profile", "1", "-p", "2", "-q", "27"],
+                                     ("slot1",)),
+    # a pan of (8, 4) px per frame: the global estimate (4 px steps) is
+    # 32 quarter samples, above the 16 that gates the global-centre search
+    "scene_640x360_pan8_p_multiref_q27": ("scene", 640, 360, 4, {"rects": 8, "pan": (8.0, 4.0)},
+                                          ["--v2", "--profile", "1", "-p", "2", "-q", "27"],
+                                          ("slot1", "glob_centre")),
+    "scene_480x272_bgop4_q32": ("scene", 480, 272, 9, {"rects": 8, "pan": (0.75, 0.25)},
+                                ["--v2", "--profile", "1", "-p", "2", "-q", "32", "-b"],
+                                ("bframe",)),
+    # the 720p configuration in full: 30 frames, 16 rectangles, sigma 2,
+    # P-only, 4-slot buffer (profile 1 searches dpb[0] and dpb[1]), QP 27
+    "config2_scene_1280x720_30f_p_multiref_q27": (
+        "scene", 1280, 720, 30, {"rects": 16, "pan": (0.0, 0.0)},
+        ["--v2", "--profile", "1", "-p", "2", "-q", "27"], ("slot1",)),
+    # the 1080p configuration's generator (48 rectangles, pan (0.75, 0.25)),
+    # 9 frames: two GOP4 ladders of hierarchical B pictures, QP 32
+    "config3_scene_1920x1080_9f_bgop4_q32": (
+        "scene", 1920, 1080, 9, {"rects": 48, "pan": (0.75, 0.25)},
+        ["--v2", "--profile", "1", "-p", "2", "-q", "32", "-b"], ("bframe",)),
+    # v1 with 4 threads and no entropy coding: WPP rows on the encoder's
+    # thread pool call tc_motion_est concurrently (8x8 stack blocks, stride 8)
+    "gradient_320x240_v1_wpp_t4": ("gradient", 320, 240, 8, {}, ["-q", "32", "-t", "4"], ()),
 }
 
 
@@ -46,57 +67,74 @@ def sha(path):
     return hashlib.sha256(open(path, "rb").read()).hexdigest()
 
 
-def run(cmd, env=None):
-    print("run:", " ".join(cmd), flush=True)
-    sys.stdout.flush()
-    return subprocess.run(cmd, check=True, capture_output=True, text=True, env=env,
-                          timeout=100)
-
-
 @pytest.fixture(scope="module")
 def clips(tmp_path_factory):
     from tcodec_amd import synth
     d = tmp_path_factory.mktemp("clips")
     out = {}
-    for name, (kind, w, h, n, flags) in CASES.items():
+    for name, (kind, w, h, n, opts, flags, sites) in CASES.items():
         if kind == "gradient":
             frames = synth.gradient_video(w, h, n, seed=1)
         else:
-            frames = synth.scene_video(w, h, n, seed=4, rects=8, pan=(0.75, 0.25))
+            frames = synth.scene_video(w, h, n, seed=4, **opts)
         p = str(d / f"{name}.yuv")
         synth.write_yuv(p, frames)
-        out[name] = (p, w, h, flags)
+        out[name] = (p, w, h, flags, sites)
     return out
 
 
+def site_lines(stderr):
+    out = []
+    for ln in stderr.splitlines():
+        if ln.startswith("TCME_SITE"):
+            out.append({k: int(v, 0) for k, v in (kv.split("=") for kv in ln.split()[1:])})
+    return out
+
+
+@pytest.mark.timeout(900)
 @pytest.mark.parametrize("name", list(CASES))
-def test_dropin_encoder_bytes_identical(clips, tmp_path, name):
-    for b in ("tcenc", "tcenc_gpu", "tcdec"):
+def test_dropin_encoder_bytes_identical_and_every_call_matches(clips, tmp_path, name):
+    for b in ("tcenc", "tcenc_gpu", "tcdec", "tcenc_hook"):
         need(os.path.join(REF, b))
-    yuv, w, h, flags = clips[name]
-    cpu, gpu = str(tmp_path / "cpu.tcv"), str(tmp_path / "gpu.tcv")
+    yuv, w, h, flags, sites = clips[name]
     args = ["-w", str(w), "-h", str(h), *flags]
-    run([os.path.join(REF, "tcenc"), *args, "-o", cpu, yuv])
-    run([os.path.join(REF, "tcenc_gpu"), *args, "-o", gpu, yuv])
-    assert os.path.getsize(cpu) > 0
-    assert sha(cpu) == sha(gpu), f"{name}: .tcv differs"
-    ycpu, ygpu = str(tmp_path / "cpu.yuv"), str(tmp_path / "gpu.yuv")
-    run([os.path.join(REF, "tcdec"), cpu, ycpu])
-    run([os.path.join(REF, "tcdec"), gpu, ygpu])
-    assert sha(ycpu) == sha(ygpu)
-
-
-@pytest.mark.parametrize("name", list(CASES))
-def test_every_encoder_call_matches(clips, tmp_path, name):
-    """Check mode: each tc_motion_est the encoder makes runs on both the
-    reference and the GPU; every {x, y, sad} must agree."""
-    need(os.path.join(REF, "tcenc_hook"))
-    yuv, w, h, flags = clips[name]
-    env = dict(os.environ, TCME_SHIM="check", TCME_LIB=LIB)
-    r = run([os.path.join(REF, "tcenc_hook"), "-w", str(w), "-h", str(h), *flags, "-o",
-             str(tmp_path / "chk.tcv"), yuv], env=env)
-    line = [ln for ln in r.stderr.splitlines() if ln.startswith("TCME_SHIM")][-1]
+    out = {k: str(tmp_path / f"{k}.tcv") for k in ("cpu", "gpu", "chk")}
+    cmds = {"cpu": [os.path.join(REF, "tcenc"), *args, "-o", out["cpu"], yuv],
+            "gpu": [os.path.join(REF, "tcenc_gpu"), *args, "-o", out["gpu"], yuv],
+            "chk": [os.path.join(REF, "tcenc_hook"), *args, "-o", out["chk"], yuv]}
+    env = {"cpu": None, "gpu": None,
+           "chk": dict(os.environ, TCME_SHIM="check", TCME_LIB=LIB)}
+    print(f"{name}: three encoders at once", flush=True)
+    sys.stdout.flush()
+    procs = {k: subprocess.Popen(c, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                                 env=env[k]) for k, c in cmds.items()}
+    res = {}
+    for k, p in procs.items():
+        so, se = p.communicate(timeout=850)
+        res[k] = (p.returncode, so, se)
+        print(f"{name}: {k} exit {p.returncode}", flush=True)
+    for k, (rc, so, se) in res.items():
+        assert rc == 0, f"{k}: {se[-2000:]}"
+    assert os.path.getsize(out["cpu"]) > 0
+    assert sha(out["cpu"]) == sha(out["gpu"]), f"{name}: .tcv differs"
+    assert sha(out["cpu"]) == sha(out["chk"])
+    # every call of the check-mode encoder agreed
+    se = res["chk"][2]
+    line = [ln for ln in se.splitlines() if ln.startswith("TCME_SHIM")][-1]
     print(line)
     fields = dict(kv.split("=") for kv in line.split()[1:])
     assert int(fields["calls"]) > 0
-    assert int(fields["mismatches"]) == 0, r.stderr[-2000:]
+    assert int(fields["mismatches"]) == 0, se[-2000:]
+    st = site_lines(se)
+    print("\n".join(str(s) for s in st))
+    assert sum(s["calls"] for s in st) == int(fields["calls"])
+    for key in sites:
+        # a site whose every call is of this kind, with calls
+        assert any(s["calls"] > 0 and s[key] == s["calls"] for s in st), (key, st)
+    # the decoder turns both streams into the same pictures
+    ycpu, ygpu = str(tmp_path / "cpu.yuv"), str(tmp_path / "gpu.yuv")
+    subprocess.run([os.path.join(REF, "tcdec"), out["cpu"], ycpu], check=True,
+                   capture_output=True, timeout=600)
+    subprocess.run([os.path.join(REF, "tcdec"), out["gpu"], ygpu], check=True,
+                   capture_output=True, timeout=600)
+    assert sha(ycpu) == sha(ygpu)
