@@ -202,8 +202,13 @@ def served_form(eng, rate: float, duration: float, budget_gib: float):
     with continuous batching under an HBM budget."""
     import torch
     from tcodec_amd import serve
-    clips = {name: serve.Clip(w, h, 3 if w > 2000 else 5, seed=77 + i)
+    # 128 device-rendered pictures per resolution: a stream (at most 121
+    # coded pictures) reads consecutive, distinct pictures
+    t0 = time.perf_counter()
+    clips = {name: serve.DeviceSceneClip(w, h, 128, seed=77 + i)
              for i, (name, (w, h)) in enumerate(serve.RES.items())}
+    torch.cuda.synchronize()
+    log(f"served: clips rendered in {time.perf_counter() - t0:.1f} s")
     reqs = serve.poisson_requests(rate, duration, seed=5, clips=clips)
     srv = serve.Server(eng, hbm_budget=int(budget_gib * (1 << 30)))
     for r in reqs:
@@ -223,7 +228,10 @@ def served_form(eng, rate: float, duration: float, budget_gib: float):
                                  "p99": round(float(np.percentile(lat1080, 99)), 2)}
                                 if lat1080.size else None),
            "ticks": srv.ticks, "hbm_budget_GiB": budget_gib,
-           "source": "moving-rectangle scene clips per resolution (5 pictures; 3 at 3840x2160)"}
+           "max_streams_per_launch": srv.max_active,
+           "admitted_while_busy": srv.admitted_while_busy,
+           "source": "moving-rectangle scene clips per resolution, 128 distinct pictures "
+                     "each (device-rendered); a stream codes consecutive pictures"}
     log(f"served: {out}")
     del clips
     torch.cuda.empty_cache()
@@ -510,7 +518,7 @@ def main():
                     help="timed time steps of the batched form (55 + 5 warm-up = the "
                          "configuration's 60 pictures per stream)")
     ap.add_argument("--serve-rate", type=float, default=40.0)
-    ap.add_argument("--serve-seconds", type=float, default=2.0,
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="arrival window of the served run (the serving config uses 120)")
     ap.add_argument("--serve-budget-gib", type=float, default=48.0)
     ap.add_argument("--launch", choices=("graph", "direct"), default="direct",

@@ -68,6 +68,11 @@ hipError_t launch_anchors(const PicDev &p, hipStream_t s);
 // followed by at least kTailSlack readable bytes.
 hipError_t launch_dropin(const PicDev *table, const tcme_call &c, const uint8_t *blk,
                          tcme_result *res, hipStream_t s);
+// n validated calls on resident pictures (calls and res device-visible,
+// normally mapped host memory), one workgroup per call, any supported size;
+// each result's status is written last, after a system-scope fence.
+hipError_t launch_list(const PicDev *table, const tcme_call *calls, int n, tcme_result *res,
+                       hipStream_t s);
 hipError_t launch_search_n(int blk, int waves, const PicDev *table, int npics,
                            const tcme_call *calls, int n, tcme_result *res, tcme_stats *stats,
                            hipStream_t s);

@@ -787,6 +787,57 @@ __global__ __launch_bounds__(64 * WPC) void me_dropin_kernel(const PicDev *__res
     }
 }
 
+// A short list of calls on resident pictures, one workgroup of 4 wavefronts
+// per call, any block size: the frame-level drop-in's searches of a call
+// that missed its cache, together with its CTU's other nodes at the same
+// displacement.  Calls are read from, and results written to, mapped host
+// memory, each result's status last after a system-scope fence, so the host
+// can take any one result as soon as its status changes (the others may
+// still be running).  The host has validated every call (reference pictures
+// with anchors, blocks inside the current picture, sizes 8/16/32/64).  An
+// 8x8 or 16x16 call is searched by the first wavefront alone (the others
+// leave at once; that search synchronises within its wavefront only).
+__global__ __launch_bounds__(256) void me_list_kernel(const PicDev *__restrict__ table,
+                                                      const tcme_call *__restrict__ calls,
+                                                      tcme_result *__restrict__ res)
+{
+    __shared__ union {
+        Shared<8, 1> s8;
+        Shared<16, 1> s16;
+        Shared<32, 4> s32;
+        Shared<64, 4> s64;
+    } sh;
+    __shared__ tcme_result r;
+    const tcme_call c = calls[blockIdx.x];
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const PicDev R = table[c.ref_pic], C = table[c.cur_pic];
+    switch (c.blk) {
+    case 8:
+        if (wave) return;
+        search<8, 1>(R, C, c, &r, nullptr, sh.s8.win, &sh.s8.red[0][0][0]);
+        break;
+    case 16:
+        if (wave) return;
+        search<16, 1>(R, C, c, &r, nullptr, sh.s16.win, &sh.s16.red[0][0][0]);
+        break;
+    case 32:
+        search<32, 4>(R, C, c, &r, nullptr, sh.s32.win, &sh.s32.red[0][0][0]);
+        break;
+    default:
+        search<64, 4>(R, C, c, &r, nullptr, sh.s64.win, &sh.s64.red[0][0][0]);
+        break;
+    }
+    // thread 0 wrote r itself (it is the call's rank-0 thread in every case)
+    if (threadIdx.x == 0) {
+        tcme_result *o = res + blockIdx.x;
+        o->mv_x = r.mv_x;
+        o->mv_y = r.mv_y;
+        o->sad = r.sad;
+        __threadfence_system();
+        o->status = r.status;
+    }
+}
+
 // Sum over each group of L consecutive lanes (L = 1, 2, 4, ..., 64, groups
 // aligned to L); every lane of a group gets its group's total.  DPP
 // butterflies: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
@@ -1550,6 +1601,14 @@ hipError_t launch_dropin(const PicDev *table, const tcme_call &c, const uint8_t 
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_list(const PicDev *table, const tcme_call *calls, int n, tcme_result *res,
+                       hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(me_list_kernel, dim3(n), dim3(256), 0, s, table, calls, res);
     return hipGetLastError();
 }
 

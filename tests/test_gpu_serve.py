@@ -58,8 +58,10 @@ def oracle_check(orc, stream, pos, got, sample=None, seed=0):
 def test_served_results_same_alone_and_in_mix(engine, orc):
     from tcodec_amd import serve, workload
     print("serve mix", flush=True)
-    clips = {"a": serve.Clip(160, 128, 5, seed=11), "b": serve.Clip(256, 144, 5, seed=12),
-             "c": serve.Clip(320, 240, 5, seed=13, kind="gradient")}
+    # 12-picture clips: a stream (at most 9 pictures, offset up to 4) never
+    # meets the same source picture as current and reference
+    clips = {"a": serve.Clip(160, 128, 12, seed=11), "b": serve.Clip(256, 144, 12, seed=12),
+             "c": serve.Clip(320, 240, 12, seed=13, kind="gradient")}
     mix = [serve.Stream(0, 160, 128, 9, "B", clips["a"], offset=0, arrival=0.0),
            serve.Stream(1, 256, 144, 7, "P", clips["b"], offset=2, arrival=0.0),
            serve.Stream(2, 320, 240, 9, "B", clips["c"], offset=1, arrival=0.02),
