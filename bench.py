@@ -303,67 +303,103 @@ def host_api_rate(eng, frames, reps: int = 2):
     return out
 
 
+ENCODER_CLIPS = (
+    # the CPU-runnable configuration: gradient, (+3,-2) px/frame, v2 P, 1 ref, QP 32
+    ("320x240 gradient, 8 frames, v2 P, 1 reference, QP 32", "gradient", 320, 240, 8,
+     ["--v2", "-q", "32"]),
+    # the 720p configuration's generator and flags, 4 frames
+    ("1280x720 scene (16 rectangles), 4 frames, v2 P, profile 1 (2 references), QP 27",
+     "scene", 1280, 720, 4, ["--v2", "--profile", "1", "-p", "2", "-q", "27"]),
+)
+
+
 def encoder_leg():
-    """The reference encoder on the CPU-runnable configuration - 320x240, 8
-    frames of the diagonal-gradient source (+3,-2) px per frame with noise,
-    v2, P-only, one reference, QP 32 - built from its sources (oracle/_ref/tcenc)
-    and linked to the drop-in (oracle/_ref/tcenc_gpu): each encoder's own clock
-    ("Time: N sec", which leaves out process start-up but, for the drop-in,
-    includes the GPU runtime's start in the first search) and process wall
-    time, and whether the .tcv bytes are identical.  Then the same encoder with
-    each tc_motion_est call timed (CLOCK_MONOTONIC around the call,
-    oracle/_ref/tcenc_hook): the reference search, then the drop-in, per call
-    after the first, and the drop-in's first call (runtime start) on its own."""
+    """The reference encoder built from its sources (oracle/_ref/tcenc) against
+    the same objects linked to the GPU backend: with the frame-level hook
+    (oracle/_ref/tcenc_gpu: each picture's predictable searches launched at
+    once, every call answered from them or searched with its CTU), and with the
+    per-call drop-in only (oracle/_ref/tcenc_gpu1).  Per clip: each encoder's
+    own clock ("Time: N sec", leaving out process start-up; the GPU encoders'
+    include the GPU runtime's start) and process wall time, whether the .tcv
+    bytes are identical, the frame-level counters (hit rate), and the ME time
+    per call measured around every tc_motion_est call (CLOCK_MONOTONIC, the
+    checker's shim: oracle/_ref/tcenc_hook for the reference and the per-call
+    drop-in, oracle/_ref/tcenc_fcheck for the frame-level path)."""
     import tempfile
 
     from tcodec_amd import synth
     ref = os.path.join(ROOT, "oracle", "_ref")
-    exes = {"cpu": os.path.join(ref, "tcenc"), "gpu_dropin": os.path.join(ref, "tcenc_gpu")}
+    lib = os.path.join(ROOT, "tcodec_amd", "lib", "libtcme.so")
+    exes = {"cpu": os.path.join(ref, "tcenc"), "gpu_frame": os.path.join(ref, "tcenc_gpu"),
+            "gpu_per_call": os.path.join(ref, "tcenc_gpu1")}
     if not all(os.path.exists(e) for e in exes.values()):
         return None
-    args = ["-w", "320", "-h", "240", "--v2", "-q", "32"]
     enc_clock = re.compile(r"Time:\s*([0-9.]+)\s*sec")
-    out = {"clip": "320x240, 8 frames, diagonal gradient + noise, v2, P-only, 1 reference, QP 32",
-           "cores": 1}
+    shim = re.compile(r"calls=(\d+) mismatches=\d+ me_seconds=([0-9.]+) "
+                      r"first_call_seconds=([0-9.]+)")
+    out = []
     with tempfile.TemporaryDirectory() as td:
-        yuv = os.path.join(td, "in.yuv")
-        synth.write_yuv(yuv, synth.gradient_video(320, 240, 8, seed=1))
-        tcv = {}
-        for name, exe in exes.items():
-            tcv[name] = os.path.join(td, name + ".tcv")
-            t0 = time.perf_counter()
-            r = subprocess.run([exe, *args, "-o", tcv[name], yuv],
-                               check=True, capture_output=True, text=True, timeout=300)
-            wall = time.perf_counter() - t0
-            m = enc_clock.search(r.stdout + r.stderr)
-            enc = float(m.group(1)) if m else None
-            out[name] = {"encode_s": enc, "frames_per_s": round(8 / enc, 2) if enc else None,
-                         "wall_s": round(wall, 3)}
-        out["tcv_identical"] = open(tcv["cpu"], "rb").read() == open(tcv["gpu_dropin"], "rb").read()
-        hook = os.path.join(ref, "tcenc_hook")
-        lib = os.path.join(ROOT, "tcodec_amd", "lib", "libtcme.so")
-        if not os.path.exists(hook):
-            return out
-        shim = re.compile(r"calls=(\d+) mismatches=\d+ me_seconds=([0-9.]+) "
-                          r"first_call_seconds=([0-9.]+)")
-        for mode, name in (("ref", "cpu"), ("gpu", "gpu_dropin")):
-            env = dict(os.environ, TCME_SHIM=mode, TCME_LIB=lib)
-            r = subprocess.run([hook, *args, "-o", os.path.join(td, mode + ".tcv"), yuv],
-                               check=True, capture_output=True, text=True, timeout=300, env=env)
-            m, e = shim.search(r.stderr), enc_clock.search(r.stdout + r.stderr)
-            if not m:
-                continue
-            calls, sec, first = int(m.group(1)), float(m.group(2)), float(m.group(3))
-            d = out[name]
-            out["me_calls"] = calls
-            d["me_s"] = round(sec, 4)
-            d["me_us_per_call"] = round((sec - first) / max(calls - 1, 1) * 1e6, 2)
-            if e:
-                d["me_share_of_encode"] = round(sec / float(e.group(1)), 3)
-            if mode == "gpu":
-                d["first_call_s"] = round(first, 4)
-                if e:
-                    d["encode_s_after_first_call"] = round(float(e.group(1)) - first, 3)
+        for label, kind, w, h, n, flags in ENCODER_CLIPS:
+            yuv = os.path.join(td, f"{w}x{h}.yuv")
+            frames = (synth.gradient_video(w, h, n, seed=1) if kind == "gradient" else
+                      synth.scene_video(w, h, n, seed=4, rects=16))
+            synth.write_yuv(yuv, frames)
+            args = ["-w", str(w), "-h", str(h), *flags]
+            row = {"clip": label, "cores": 1}
+            tcv = {}
+            for name, exe in exes.items():
+                tcv[name] = os.path.join(td, name + ".tcv")
+                env = dict(os.environ, TCME_HOOK_STATS="1")
+                t0 = time.perf_counter()
+                r = subprocess.run([exe, *args, "-o", tcv[name], yuv], check=True,
+                                   capture_output=True, text=True, timeout=600, env=env)
+                wall = time.perf_counter() - t0
+                m = enc_clock.search(r.stdout + r.stderr)
+                enc = float(m.group(1)) if m else None
+                d = {"encode_s": enc, "frames_per_s": round(n / enc, 2) if enc else None,
+                     "wall_s": round(wall, 3)}
+                fl = [ln for ln in r.stderr.splitlines() if ln.startswith("TCME_FRAME")]
+                if fl:
+                    fs = {k: float(v) for k, v in (kv.split("=") for kv in fl[-1].split()[1:])}
+                    calls = max(fs["calls"], 1)
+                    d["frame_level"] = {
+                        "calls": int(fs["calls"]),
+                        "hit_rate": round((fs["hits_spec"] + fs["hits_batch"]) / calls, 4),
+                        "hits_frame_start": int(fs["hits_spec"]),
+                        "hits_miss_batches": int(fs["hits_batch"]), "misses": int(fs["misses"]),
+                        "searches_at_frame_start": int(fs["spec_calls"]),
+                        "searches_in_miss_batches": int(fs["batch_calls"]),
+                        "host_s": {"frame_begin": round(fs["begin_s"], 4),
+                                   "wait_frame_start": round(fs["wait_s"], 4),
+                                   "misses": round(fs["miss_s"], 4)}}
+                row[name] = d
+            row["tcv_identical"] = (open(tcv["cpu"], "rb").read() ==
+                                    open(tcv["gpu_frame"], "rb").read() ==
+                                    open(tcv["gpu_per_call"], "rb").read())
+            timed = (("ref", "cpu", "tcenc_hook"), ("gpu", "gpu_per_call", "tcenc_hook"),
+                     ("gpu", "gpu_frame", "tcenc_fcheck"))
+            for mode, name, exe in timed:
+                path = os.path.join(ref, exe)
+                if not os.path.exists(path):
+                    continue
+                env = dict(os.environ, TCME_SHIM=mode, TCME_LIB=lib)
+                r = subprocess.run([path, *args, "-o", os.path.join(td, "t.tcv"), yuv],
+                                   check=True, capture_output=True, text=True, timeout=600,
+                                   env=env)
+                m = shim.search(r.stderr)
+                if not m:
+                    continue
+                calls, sec, first = int(m.group(1)), float(m.group(2)), float(m.group(3))
+                row["me_calls"] = calls
+                d = row[name]
+                d["me_s"] = round(sec, 4)
+                d["me_us_per_call"] = round((sec - first) / max(calls - 1, 1) * 1e6, 2)
+                d["me_first_call_s"] = round(first, 4)
+            cpu_s, gpu_s = row["cpu"]["encode_s"], row["gpu_frame"]["encode_s"]
+            if cpu_s and gpu_s:
+                row["frame_level_speedup"] = round(cpu_s / gpu_s, 3)
+            out.append(row)
+            log(f"encoder: {row}")
     return out
 
 

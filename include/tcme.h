@@ -34,10 +34,15 @@ typedef struct tcme_mv { int32_t x, y; } tcme_mv;
 /* Replaces tc_motion_est, reference src/motion.c:384-389 (prototype
  * include/tcodec_common.h:454-459).  Returns the absolute quarter-pel
  * position of the best n x n block and, if best_sad_out is not NULL, its SAD.
- * blk_size must be 8, 16, 32 or 64 (every size the reference encoder uses). */
+ * blk_size must be 8, 16, 32 or 64 (every size the reference encoder uses).
+ * A translation unit that also includes the reference's tcodec_common.h,
+ * which declares the same symbol with its own (layout-identical) types,
+ * defines TCME_HIDE_DROPIN before including this header. */
+#ifndef TCME_HIDE_DROPIN
 tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, int ref_h,
                       const uint8_t *orig, int orig_stride, int center_x, int center_y,
                       int blk_size, int search_range, int32_t *best_sad_out);
+#endif
 
 /* ---- batch API ---------------------------------------------------------- */
 
@@ -217,6 +222,100 @@ int  tcme_last_timing(tcme_ctx *ctx, float *ms, int max);
  * synchronises the stream. */
 int  tcme_search_host(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,
                       tcme_stats *stats, void *stream);
+
+/* ---- frame-level drop-in ------------------------------------------------ */
+
+/* The reference encoder calls tc_motion_est (src/encoder.c:807, 834, 928,
+ * 982, 1004, 1035) once per quadtree node and reference, sequentially from
+ * its RDO, on planes that do not change during a frame (the current picture
+ * is staged at src/encoder.c:2631, the picture buffer is updated only after
+ * the frame, src/encoder.c:3085-3097).  The frame-level form replaces those
+ * calls (same arguments, same results) with:
+ *   tcme_frame_begin        once per coded picture: its planes and the
+ *                           searches whose centres are known at frame start,
+ *                           which are all searched at once, in one launch;
+ *   tcme_frame_motion_est   every call: answered from those results when it
+ *                           is one of them; otherwise searched at once on the
+ *                           resident planes, in one launch with the other
+ *                           nodes of its 64x64 CTU against the same
+ *                           references at the same displacement (the
+ *                           encoder's predictor, src/encoder.c:405-427, is
+ *                           reset per CTU, src/encoder.c:1425-1428, so these
+ *                           are its likely next calls).
+ * It has no counterpart in the reference: include the hook file
+ * tcodec_amd/glue/tcme_encoder_hook.c to drive it from the unmodified
+ * encoder.  Every answer equals tc_motion_est's. */
+typedef struct tcme_frame tcme_frame;
+
+/* Speculative centres of one set of searches. */
+enum { TCME_CENTRE_OFFSET = 0, TCME_CENTRE_GRID = 1 };
+
+/* One 8x8 cell of a displacement grid (quarter samples), row-major. */
+typedef struct tcme_cell {
+    int16_t dx, dy;
+    uint8_t unavailable;  /* nonzero: no search for nodes whose top-left cell this is */
+    uint8_t reserved;
+} tcme_cell;
+
+/* Every in-frame quadtree node of the sizes in `sizes` (bit k: 8 << k)
+ * against reference `ref` with search range `sr`, centred as the reference
+ * encoder centres a call on a quarter-sample displacement d: the node at
+ * (px, py) gets the full-sample centre ((4 px + d.x) >> 2, (4 py + d.y) >> 2).
+ * OFFSET: d = (dx, dy) for every node (0: the block position, as at the
+ * predictor sites when the predictor is zero; the global-motion estimate, as
+ * at src/encoder.c:1000-1004).  GRID: d = the cell of grid_w x grid_h `grid`
+ * at (px/8, py/8) (the previous frame's displacements, as at
+ * src/encoder.c:1023-1035); nodes outside the grid or on an unavailable cell
+ * get no search.  The grid is read during tcme_frame_begin only. */
+typedef struct tcme_spec {
+    int32_t ref, sr, sizes, mode;
+    int32_t dx, dy;
+    const tcme_cell *grid;
+    int32_t grid_w, grid_h;
+} tcme_spec;
+
+/* Counters of a frame object since its creation. */
+typedef struct tcme_frame_stats {
+    int64_t frames;       /* tcme_frame_begin calls */
+    int64_t spec_calls;   /* searches launched at frame start */
+    int64_t calls;        /* tcme_frame_motion_est calls */
+    int64_t hits_spec;    /* answered from the frame-start searches */
+    int64_t hits_batch;   /* answered from an earlier miss's launch */
+    int64_t misses;       /* searched at once (one launch each) */
+    int64_t batch_calls;  /* searches those launches made */
+    int64_t uncached;     /* not on the declared planes: handed to tc_motion_est */
+    double begin_s;       /* host seconds inside tcme_frame_begin */
+    double wait_s;        /* host seconds waiting for the frame-start results */
+    double miss_s;        /* host seconds inside misses */
+} tcme_frame_stats;
+
+/* Starts the GPU runtime and the drop-in's device context on a background
+ * thread and returns at once, so that an encoder can overlap the runtime's
+ * start with its first (intra) picture.  Later calls do nothing. */
+void tcme_prepare(void);
+
+int  tcme_frame_create(tcme_frame **out);
+void tcme_frame_destroy(tcme_frame *f);
+/* Declares the planes of the picture about to be coded (host memory, w x h
+ * luma; refs[i] may be NULL) and launches the searches `specs` describe.  The
+ * planes must not change until the next tcme_frame_begin on this object.
+ * Returns once the work is queued. */
+int  tcme_frame_begin(tcme_frame *f, const uint8_t *cur, int cur_stride, int w, int h,
+                      const uint8_t *const *refs, const int *ref_strides, int nrefs,
+                      const tcme_spec *specs, int nspecs);
+/* Ends the declared frame (its planes may change from now on): until the
+ * next tcme_frame_begin every call is handed to tc_motion_est. */
+int  tcme_frame_end(tcme_frame *f);
+/* tc_motion_est for the declared frame: the arguments and result of
+ * reference src/motion.c:384-389.  A call that is not a node of the declared
+ * planes (another reference, a block outside the current picture such as the
+ * v1 coder's stack copy, an unaligned position) is handed to tc_motion_est.
+ * Thread-safe (calls on one object are serialised). */
+tcme_mv tcme_frame_motion_est(tcme_frame *f, const uint8_t *ref, int ref_stride, int ref_w,
+                              int ref_h, const uint8_t *orig, int orig_stride, int center_x,
+                              int center_y, int blk_size, int search_range,
+                              int32_t *best_sad_out);
+int  tcme_frame_get_stats(tcme_frame *f, tcme_frame_stats *st);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,12 @@
  * then supplies both symbols.  The ME call site contract it sits on is
  * src/motion.c:384-389 / include/tcodec_common.h:454-459.
  *
+ * Built with -DTCME_SHIM_FRAME (oracle/_ref/tcenc_fcheck) the file instead
+ * supplies tcme_shim_motion_est, the name encoder.c's calls get on its
+ * compile line there, and the GPU side of every call is the frame-level hook
+ * of tcodec_amd/glue/tcme_encoder_hook.c (linked with libtcme.so) instead of
+ * the library named by TCME_LIB.
+ *
  * TCME_SHIM selects what each tc_motion_est call does:
  *   ref    (default) run the reference function, time it (CLOCK_MONOTONIC)
  *   gpu    run tc_motion_est from the library named by TCME_LIB (libtcme.so)
@@ -42,6 +48,19 @@ tc_encoder_t *tc_encoder_create(const tc_config_t *config);
 
 typedef tc_mv_s (*me_fn_t)(const tc_pixel_t *, int, int, int, const tc_pixel_t *, int,
                            int, int, int, int, tc_sad_t *);
+
+#ifdef TCME_SHIM_FRAME
+#define SHIM_ENTRY tcme_shim_motion_est
+tc_mv_s tcme_hook_motion_est(const tc_pixel_t *ref, int ref_stride, int ref_w, int ref_h,
+                             const tc_pixel_t *orig, int orig_stride, int center_x,
+                             int center_y, int blk_size, int search_range,
+                             tc_sad_t *best_sad_out);
+tc_mv_s SHIM_ENTRY(const tc_pixel_t *ref, int ref_stride, int ref_w, int ref_h,
+                   const tc_pixel_t *orig, int orig_stride, int center_x, int center_y,
+                   int blk_size, int search_range, tc_sad_t *best_sad_out);
+#else
+#define SHIM_ENTRY tc_motion_est
+#endif
 
 enum { MODE_REF, MODE_GPU, MODE_CHECK, MODE_TRACE };
 static int g_mode = -1;
@@ -125,7 +144,10 @@ static void shim_init(void)
     if (m && !strcmp(m, "gpu")) g_mode = MODE_GPU;
     else if (m && !strcmp(m, "check")) g_mode = MODE_CHECK;
     else if (m && !strcmp(m, "trace")) g_mode = MODE_TRACE;
-    if (g_mode == MODE_GPU || g_mode == MODE_CHECK) {
+#ifdef TCME_SHIM_FRAME
+    if (g_mode == MODE_GPU || g_mode == MODE_CHECK) g_gpu_fn = tcme_hook_motion_est;
+#endif
+    if (!g_gpu_fn && (g_mode == MODE_GPU || g_mode == MODE_CHECK)) {
         const char *lib = getenv("TCME_LIB");
         void *h = lib ? dlopen(lib, RTLD_NOW | RTLD_LOCAL) : NULL;
         g_gpu_fn = h ? (me_fn_t)dlsym(h, "tc_motion_est") : NULL;
@@ -204,10 +226,10 @@ static void trace_call(const tc_pixel_t *ref, int ref_stride, int w, int h,
     fwrite(rec, sizeof(int), 16, g_trace);
 }
 
-tc_mv_s tc_motion_est(const tc_pixel_t *ref, int ref_stride, int ref_w, int ref_h,
-                      const tc_pixel_t *orig, int orig_stride, int center_x,
-                      int center_y, int blk_size, int search_range,
-                      tc_sad_t *best_sad_out)
+tc_mv_s SHIM_ENTRY(const tc_pixel_t *ref, int ref_stride, int ref_w, int ref_h,
+                   const tc_pixel_t *orig, int orig_stride, int center_x,
+                   int center_y, int blk_size, int search_range,
+                   tc_sad_t *best_sad_out)
 {
     pthread_once(&g_once, shim_init);
     const long call = __atomic_add_fetch(&g_calls, 1, __ATOMIC_RELAXED);

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "tcme_internal.h"
@@ -744,6 +745,8 @@ struct DropIn {
 
 DropIn g_drop;
 constexpr int kMirrors = 8;
+std::once_flag g_drop_once;
+std::string g_drop_init_err;  // set by a failed start (dropin_init_body)
 
 [[noreturn]] void die(const char *what)
 {
@@ -751,12 +754,33 @@ constexpr int kMirrors = 8;
     abort();
 }
 
-void dropin_init()
+void dropin_init_body();
+
+void dropin_ensure()
+{
+    std::call_once(g_drop_once, dropin_init_body);
+    if (!g_drop_init_err.empty()) {
+        fail(TCME_E_HIP, "%s", g_drop_init_err.c_str());
+        die("drop-in");
+    }
+}
+
+// The drop-in's device context, pinned staging and stream.  Run once (it may
+// run on tcme_prepare's thread); failures are kept in init_err and reported
+// by the first call that needs the GPU (dropin_ensure).
+void dropin_init_body()
 {
     DropIn &d = g_drop;
     const char *dev = getenv("TCME_DEVICE");
-    if (tcme_create(&d.ctx, dev ? atoi(dev) : 0, 64) != TCME_OK) die("no usable GPU");
-    if (hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess) die("stream");
+    if (tcme_create(&d.ctx, dev ? atoi(dev) : 0, 64) != TCME_OK) {
+        g_drop_init_err = std::string("no usable GPU: ") + tcme_last_error();
+        d.ctx = nullptr;
+        return;
+    }
+    if (hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess) {
+        g_drop_init_err = "drop-in stream";
+        return;
+    }
     // The kernel reads each 4-byte unit of the block as the aligned dword pair
     // that covers it, so the last unit of a 64x64 block also reads bytes
     // 4096-4099: the buffer carries the same tail slack as a device picture.
@@ -767,8 +791,10 @@ void dropin_init()
         hipHostMalloc(reinterpret_cast<void **>(&d.h_res), sizeof(tcme_result),
                       hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer(&db, d.h_blk, 0) != hipSuccess ||
-        hipHostGetDevicePointer(&dr, d.h_res, 0) != hipSuccess)
-        die("drop-in staging");
+        hipHostGetDevicePointer(&dr, d.h_res, 0) != hipSuccess) {
+        g_drop_init_err = "drop-in staging";
+        return;
+    }
     d.d_blk = static_cast<const uint8_t *>(db);
     d.d_res = static_cast<tcme_result *>(dr);
 }
@@ -796,6 +822,7 @@ int mirror_for(const uint8_t *ref, int stride, int w, int h, int x0, int y0, int
             m = &*std::min_element(d.mirrors.begin(), d.mirrors.end(),
                                    [](const Mirror &a, const Mirror &b) { return a.last_use < b.last_use; });
         }
+        std::lock_guard<std::mutex> gc(d.ctx->mu);  // the picture table is shared
         if (m->pic >= 0 && (m->w != w || m->h != h)) {
             if (hipStreamSynchronize(d.stream) != hipSuccess) die("stream synchronize");
             pic_free(d.ctx, m->pic);
@@ -828,8 +855,8 @@ TCME_API tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, in
                                int blk_size, int search_range, int32_t *best_sad_out)
 {
     DropIn &d = g_drop;
+    dropin_ensure();
     std::lock_guard<std::mutex> g(d.mu);
-    if (!d.ctx) dropin_init();
     if (blk_size != 8 && blk_size != 16 && blk_size != 32 && blk_size != 64) {
         fail(TCME_E_BLK, "block size %d (supported: 8, 16, 32, 64)", blk_size);
         die("tc_motion_est");
@@ -886,4 +913,550 @@ TCME_API tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, in
     if (best_sad_out) *best_sad_out = hr->sad;
     tcme_mv mv = {hr->mv_x, hr->mv_y};
     return mv;
+}
+
+// ---------------------------------------------------------------------------
+// Frame-level drop-in (tcme_frame_*)
+//
+// An encoder declares each frame's planes once (tcme_frame_begin) together
+// with the searches its call sites will make at centres known at frame start.
+// Those are searched at once, in one launch, and every later call with
+// exactly the same arguments is answered from their results.  A call that
+// finds no result is searched at once on the resident planes, in one launch
+// together with the other nodes of its CTU at the same displacement (the
+// reference's predictor is CTU-local, so they are the likely next calls).
+// Results are pure functions of the arguments, so the answers are the
+// reference's whichever way they were found.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kFrameRefs = 8;       // planes a frame may declare
+constexpr int kRing = 1 << 15;      // miss-batch result slots
+constexpr int kBatchMax = 4 * 85 + 1;
+constexpr uint32_t kRingBit = 0x80000000u;
+
+struct FrameRef {
+    const uint8_t *host = nullptr;
+    int stride = 0, pic = -1;
+    std::vector<uint8_t> shadow;    // w*h bytes the device picture holds
+    uint64_t last_use = 0;
+};
+
+// The search key of a call on the frame's planes: reference slot, block
+// size, node position (multiples of 8), displacement of the centre from the
+// node (|d| < 2048) and search range (< 16384).  Never 0.
+bool frame_key(int r, int px, int py, int cx, int cy, int n, int sr, uint64_t &k)
+{
+    const int code = n == 8 ? 0 : n == 16 ? 1 : n == 32 ? 2 : n == 64 ? 3 : -1;
+    if (code < 0 || r < 0 || r > 3 || px < 0 || py < 0 || (px & 7) || (py & 7) ||
+        px >= tcme::kMaxPicSide || py >= tcme::kMaxPicSide || sr < 0 || sr > 16383)
+        return false;
+    const int64_t dx = static_cast<int64_t>(cx) - px + 2048, dy = static_cast<int64_t>(cy) - py + 2048;
+    if (dx < 1 || dx > 4095 || dy < 1 || dy > 4095) return false;
+    k = static_cast<uint64_t>(r) | static_cast<uint64_t>(code) << 2 |
+        static_cast<uint64_t>(px >> 3) << 4 | static_cast<uint64_t>(py >> 3) << 15 |
+        static_cast<uint64_t>(dx) << 26 | static_cast<uint64_t>(dy) << 38 |
+        static_cast<uint64_t>(sr) << 50;
+    return true;
+}
+
+uint32_t key_hash(uint64_t k)
+{
+    k ^= k >> 29;
+    k *= 0xBF58476D1CE4E5B9ull;
+    k ^= k >> 32;
+    return static_cast<uint32_t>(k);
+}
+
+double seconds_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct tcme_frame {
+    std::mutex mu;
+    int w = 0, h = 0;
+    int cur_pic = -1;
+    const uint8_t *cur_host = nullptr;
+    int cur_stride = 0;
+    std::vector<FrameRef> pool;     // device pictures of references, reused by content
+    uint64_t tick = 0;
+    int nrefs = 0;
+    const uint8_t *ref_host[kFrameRefs] = {};
+    int ref_pic[kFrameRefs] = {};
+    std::vector<tcme_spec> specs;   // grids not kept
+    int active = 0;                 // a frame is declared
+    // frame-start searches: host list (pinned), device list and results,
+    // results copied back to pinned memory
+    tcme_call *h_calls = nullptr;
+    tcme_result *h_res = nullptr;
+    tcme_call *d_calls = nullptr;
+    tcme_result *d_res = nullptr;
+    size_t cap = 0;
+    int nspec = 0;
+    std::vector<uint64_t> spec_key;
+    hipEvent_t spec_done = nullptr;
+    bool spec_wait = false;
+    // miss batches: calls and results in mapped pinned memory
+    tcme_call *ring_calls = nullptr, *ring_calls_d = nullptr;
+    tcme_result *ring_res = nullptr, *ring_res_d = nullptr;
+    std::vector<uint64_t> ring_key;
+    int ring_pos = 0;
+    hipStream_t st[2] = {nullptr, nullptr};
+    int next = 0;
+    // open-addressing table: key -> result index (kRingBit: a ring slot)
+    std::vector<uint64_t> hkey;
+    std::vector<uint32_t> hval;
+    uint32_t hmask = 0;
+    size_t hcount = 0;
+    tcme_frame_stats stats{};
+};
+
+namespace {
+
+bool frame_find(const tcme_frame *f, uint64_t k, uint32_t &v)
+{
+    if (!f->hmask) return false;
+    for (uint32_t i = key_hash(k) & f->hmask;; i = (i + 1) & f->hmask) {
+        if (f->hkey[i] == k) { v = f->hval[i]; return true; }
+        if (f->hkey[i] == 0) return false;
+    }
+}
+
+// Inserts k -> v; a present k keeps its value unless `replace`.  Returns
+// whether k was absent.
+bool frame_insert(tcme_frame *f, uint64_t k, uint32_t v, bool replace = false)
+{
+    for (uint32_t i = key_hash(k) & f->hmask;; i = (i + 1) & f->hmask) {
+        if (f->hkey[i] == k) {
+            if (replace) f->hval[i] = v;
+            return false;
+        }
+        if (f->hkey[i] == 0) {
+            f->hkey[i] = k;
+            f->hval[i] = v;
+            f->hcount++;
+            return true;
+        }
+    }
+}
+
+// A live result index for k: a frame-start search, or a ring slot that still
+// holds k's call (a slot reused by a later batch no longer does).
+bool frame_lookup(const tcme_frame *f, uint64_t k, uint32_t &v)
+{
+    return frame_find(f, k, v) && (!(v & kRingBit) || f->ring_key[v & ~kRingBit] == k);
+}
+
+void frame_table_reset(tcme_frame *f, size_t entries)
+{
+    size_t cap = 1024;
+    while (cap < 2 * (entries + kRing)) cap <<= 1;
+    if (cap != f->hkey.size()) {
+        f->hkey.assign(cap, 0);
+        f->hval.assign(cap, 0);
+    } else {
+        std::fill(f->hkey.begin(), f->hkey.end(), 0);
+    }
+    f->hmask = static_cast<uint32_t>(cap - 1);
+    f->hcount = 0;
+}
+
+// Rebuilds the table from the frame-start searches and the live ring slots
+// (stale ring keys are dropped).
+void frame_table_rebuild(tcme_frame *f)
+{
+    std::fill(f->hkey.begin(), f->hkey.end(), 0);
+    f->hcount = 0;
+    for (int i = 0; i < f->nspec; ++i) frame_insert(f, f->spec_key[i], static_cast<uint32_t>(i));
+    for (int i = 0; i < kRing; ++i)
+        if (f->ring_key[i]) frame_insert(f, f->ring_key[i], kRingBit | i);
+}
+
+int frame_alloc_lists(tcme_frame *f, size_t n)
+{
+    if (n <= f->cap) return TCME_OK;
+    HIP_TRY(hipDeviceSynchronize());
+    if (f->h_calls) (void)hipHostFree(f->h_calls);
+    if (f->h_res) (void)hipHostFree(f->h_res);
+    (void)hipFree(f->d_calls);
+    (void)hipFree(f->d_res);
+    f->h_calls = nullptr; f->h_res = nullptr; f->d_calls = nullptr; f->d_res = nullptr;
+    f->cap = 0;
+    const size_t cap = std::max<size_t>(n, 4096);
+    if (hipHostMalloc(reinterpret_cast<void **>(&f->h_calls), cap * sizeof(tcme_call), 0) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&f->h_res), cap * sizeof(tcme_result), 0) != hipSuccess ||
+        hipMalloc(&f->d_calls, cap * sizeof(tcme_call)) != hipSuccess ||
+        hipMalloc(&f->d_res, cap * sizeof(tcme_result)) != hipSuccess)
+        return fail(TCME_E_NOMEM, "frame search lists for %zu calls", cap);
+    f->cap = cap;
+    return TCME_OK;
+}
+
+// The pool picture holding host plane `ref` (exactly its bytes), uploading
+// it into the least recently used picture not in `busy` when none does.
+int frame_ref_pic(tcme_frame *f, const uint8_t *ref, int stride, const int *busy, int nbusy,
+                  hipStream_t s)
+{
+    tcme_ctx *c = g_drop.ctx;
+    const size_t w = static_cast<size_t>(f->w);
+    for (FrameRef &e : f->pool) {
+        if (e.host != ref || e.stride != stride || e.pic < 0) continue;
+        bool same = true;
+        for (int y = 0; y < f->h && same; ++y)
+            same = memcmp(e.shadow.data() + y * w, ref + static_cast<size_t>(y) * stride, w) == 0;
+        if (same) {
+            e.last_use = ++f->tick;
+            return e.pic;
+        }
+    }
+    FrameRef *pick = nullptr;
+    for (FrameRef &e : f->pool) {
+        bool used = false;
+        for (int i = 0; i < nbusy; ++i) used |= e.pic == busy[i];
+        if (!used && (!pick || e.last_use < pick->last_use)) pick = &e;
+    }
+    if (!pick) {
+        f->pool.emplace_back();
+        pick = &f->pool.back();
+        pick->pic = pic_alloc(c, f->w, f->h, true);
+        if (pick->pic < 0) return pick->pic;
+    }
+    pick->host = ref;
+    pick->stride = stride;
+    pick->shadow.resize(w * f->h);
+    for (int y = 0; y < f->h; ++y)
+        memcpy(pick->shadow.data() + y * w, ref + static_cast<size_t>(y) * stride, w);
+    if (int e = tcme_pic_upload(c, pick->pic, pick->shadow.data(), f->w, s)) return e;
+    pick->last_use = ++f->tick;
+    return pick->pic;
+}
+
+void frame_release_pics(tcme_frame *f)
+{
+    tcme_ctx *c = g_drop.ctx;
+    if (!c) return;
+    for (FrameRef &e : f->pool)
+        if (e.pic >= 0) pic_free(c, e.pic);
+    f->pool.clear();
+    if (f->cur_pic >= 0) pic_free(c, f->cur_pic);
+    f->cur_pic = -1;
+}
+
+// Waits until ring slot i holds its result (status written last by the
+// kernel); past 2 ms without it, synchronises the streams, which also
+// reports a failed launch.
+const tcme_result &ring_wait(tcme_frame *f, int i)
+{
+    volatile tcme_result *r = f->ring_res + i;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spin = 1; r->status == TCME_E_HIP; ++spin) {
+        if ((spin & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+            if (hipStreamSynchronize(f->st[0]) != hipSuccess ||
+                hipStreamSynchronize(f->st[1]) != hipSuccess)
+                die("frame search");
+            if (r->status == TCME_E_HIP) {
+                fail(TCME_E_HIP, "a frame search never wrote its result");
+                die("frame search");
+            }
+            break;
+        }
+        __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return f->ring_res[i];
+}
+
+}  // namespace
+
+TCME_API void tcme_prepare(void)
+{
+    static std::once_flag started;
+    std::call_once(started, [] { std::thread([] { std::call_once(g_drop_once, dropin_init_body); }).detach(); });
+}
+
+TCME_API int tcme_frame_create(tcme_frame **out)
+{
+    if (!out) return fail(TCME_E_ARG, "null out");
+    *out = nullptr;
+    dropin_ensure();
+    tcme_frame *f = new tcme_frame();
+    if (hipSetDevice(g_drop.ctx->device) != hipSuccess ||
+        hipEventCreateWithFlags(&f->spec_done, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&f->st[0], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&f->st[1], hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&f->ring_calls), kRing * sizeof(tcme_call),
+                      hipHostMallocCoherent) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&f->ring_res), kRing * sizeof(tcme_result),
+                      hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void **>(&f->ring_calls_d), f->ring_calls, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void **>(&f->ring_res_d), f->ring_res, 0) != hipSuccess) {
+        tcme_frame_destroy(f);
+        return fail(TCME_E_HIP, "frame streams and miss-batch memory");
+    }
+    f->ring_key.assign(kRing, 0);
+    *out = f;
+    return TCME_OK;
+}
+
+TCME_API void tcme_frame_destroy(tcme_frame *f)
+{
+    if (!f) return;
+    if (g_drop.ctx) {
+        (void)hipSetDevice(g_drop.ctx->device);
+        for (hipStream_t s : f->st)
+            if (s) (void)hipStreamSynchronize(s);
+        std::lock_guard<std::mutex> g(g_drop.ctx->mu);
+        frame_release_pics(f);
+    }
+    for (hipStream_t s : f->st)
+        if (s) (void)hipStreamDestroy(s);
+    if (f->spec_done) (void)hipEventDestroy(f->spec_done);
+    if (f->h_calls) (void)hipHostFree(f->h_calls);
+    if (f->h_res) (void)hipHostFree(f->h_res);
+    (void)hipFree(f->d_calls);
+    (void)hipFree(f->d_res);
+    if (f->ring_calls) (void)hipHostFree(f->ring_calls);
+    if (f->ring_res) (void)hipHostFree(f->ring_res);
+    delete f;
+}
+
+TCME_API int tcme_frame_begin(tcme_frame *f, const uint8_t *cur, int cur_stride, int w, int h,
+                              const uint8_t *const *refs, const int *ref_strides, int nrefs,
+                              const tcme_spec *specs, int nspecs)
+{
+    if (!f) return fail(TCME_E_ARG, "null frame");
+    if (!cur || w <= 0 || h <= 0 || w > tcme::kMaxPicSide || h > tcme::kMaxPicSide || cur_stride < w)
+        return fail(TCME_E_ARG, "bad current plane %dx%d stride %d", w, h, cur_stride);
+    if (nrefs < 0 || nrefs > kFrameRefs || (nrefs > 0 && (!refs || !ref_strides)))
+        return fail(TCME_E_ARG, "bad reference list (%d, at most %d)", nrefs, kFrameRefs);
+    if (nspecs < 0 || (nspecs > 0 && !specs)) return fail(TCME_E_ARG, "bad spec list");
+    for (int i = 0; i < nrefs; ++i)
+        if (refs[i] && ref_strides[i] < w) return fail(TCME_E_ARG, "reference %d stride", i);
+    for (int i = 0; i < nspecs; ++i) {
+        const tcme_spec &p = specs[i];
+        if (p.ref < 0 || p.ref >= nrefs || p.ref > 3 || !refs[p.ref] || p.sr < 0 || p.sr > 16383 ||
+            (p.sizes & ~15) || (p.mode != TCME_CENTRE_OFFSET && p.mode != TCME_CENTRE_GRID) ||
+            (p.mode == TCME_CENTRE_GRID && (!p.grid || p.grid_w < 0 || p.grid_h < 0)) ||
+            p.dx < -8188 || p.dx > 8188 || p.dy < -8188 || p.dy > 8188)
+            return fail(TCME_E_ARG, "bad spec %d", i);
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> g(f->mu);
+    tcme_ctx *c = g_drop.ctx;
+    HIP_TRY(hipSetDevice(c->device));
+    // the previous frame's work may still read the pictures
+    HIP_TRY(hipStreamSynchronize(f->st[0]));
+    HIP_TRY(hipStreamSynchronize(f->st[1]));
+    std::lock_guard<std::mutex> gc(c->mu);
+    if (w != f->w || h != f->h) {
+        frame_release_pics(f);
+        f->w = w;
+        f->h = h;
+    }
+    if (f->cur_pic < 0) {
+        f->cur_pic = pic_alloc(c, w, h, false);
+        if (f->cur_pic < 0) return f->cur_pic;
+    }
+    hipStream_t s = f->st[0];
+    if (int e = tcme_pic_upload(c, f->cur_pic, cur, cur_stride, s)) return e;
+    f->cur_host = cur;
+    f->cur_stride = cur_stride;
+    f->nrefs = nrefs;
+    int busy[kFrameRefs];
+    int nbusy = 0;
+    for (int i = 0; i < nrefs; ++i) {
+        f->ref_host[i] = refs[i];
+        f->ref_pic[i] = -1;
+        if (!refs[i]) continue;
+        const int pic = frame_ref_pic(f, refs[i], ref_strides[i], busy, nbusy, s);
+        if (pic < 0) return pic;
+        f->ref_pic[i] = pic;
+        busy[nbusy++] = pic;
+    }
+    // the frame-start searches, deduplicated by key
+    size_t upper = 0;
+    for (int i = 0; i < nspecs; ++i)
+        for (int k = 0; k < 4; ++k)
+            if (specs[i].sizes >> k & 1) upper += static_cast<size_t>(w / (8 << k)) * (h / (8 << k));
+    if (int e = frame_alloc_lists(f, upper + 1)) return e;
+    frame_table_reset(f, upper);
+    f->spec_key.resize(upper + 1);
+    std::fill(f->ring_key.begin(), f->ring_key.end(), 0);
+    f->ring_pos = 0;
+    int n = 0;
+    for (int i = 0; i < nspecs; ++i) {
+        const tcme_spec &p = specs[i];
+        for (int k = 0; k < 4; ++k) {
+            if (!(p.sizes >> k & 1)) continue;
+            const int bs = 8 << k;
+            for (int py = 0; py + bs <= h; py += bs)
+                for (int px = 0; px + bs <= w; px += bs) {
+                    int dx = p.dx, dy = p.dy;
+                    if (p.mode == TCME_CENTRE_GRID) {
+                        const int gx = px >> 3, gy = py >> 3;
+                        if (gx >= p.grid_w || gy >= p.grid_h) continue;
+                        const tcme_cell &cell = p.grid[static_cast<size_t>(gy) * p.grid_w + gx];
+                        if (cell.unavailable) continue;
+                        dx = cell.dx;
+                        dy = cell.dy;
+                    }
+                    const int cx = (4 * px + dx) >> 2, cy = (4 * py + dy) >> 2;
+                    uint64_t key;
+                    if (!frame_key(p.ref, px, py, cx, cy, bs, p.sr, key)) continue;
+                    if (!frame_insert(f, key, static_cast<uint32_t>(n))) continue;
+                    f->spec_key[n] = key;
+                    f->h_calls[n++] = tcme_call{f->ref_pic[p.ref], f->cur_pic, px, py, cx, cy, bs, p.sr};
+                }
+        }
+    }
+    f->nspec = n;
+    f->specs.assign(specs, specs + nspecs);
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(f->d_calls, f->h_calls, n * sizeof(tcme_call), hipMemcpyHostToDevice, s));
+        if (int e = dispatch(c, f->d_calls, n, c->tiling, f->d_res, nullptr, s)) return e;
+        HIP_TRY(hipMemcpyAsync(f->h_res, f->d_res, n * sizeof(tcme_result), hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipEventRecord(f->spec_done, s));
+    f->spec_wait = true;
+    f->active = 1;
+    f->stats.frames++;
+    f->stats.spec_calls += n;
+    f->stats.begin_s += seconds_since(t0);
+    return TCME_OK;
+}
+
+namespace {
+
+// One frame call that found no result: searched now, with the other nodes of
+// its CTU against the same references at the same displacement.
+tcme_result frame_miss(tcme_frame *f, int r, int px, int py, int cx, int cy, int n, int sr)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    // the references the frame searches at the predictor centre with this range
+    int refs[kFrameRefs];
+    int nr = 0;
+    refs[nr++] = r;
+    for (const tcme_spec &p : f->specs)
+        if (p.mode == TCME_CENTRE_OFFSET && p.dx == 0 && p.dy == 0 && p.sr == sr &&
+            std::find(refs, refs + nr, p.ref) == refs + nr)
+            refs[nr++] = p.ref;
+    if (f->ring_pos + kBatchMax > kRing) {
+        // wrap: every earlier batch must be done before its slots are reused
+        if (hipStreamSynchronize(f->st[0]) != hipSuccess || hipStreamSynchronize(f->st[1]) != hipSuccess)
+            die("frame search");
+        f->ring_pos = 0;
+    }
+    const int base = f->ring_pos;
+    int k = 0;
+    auto add = [&](int rr, int x, int y, int bs, bool first) {
+        uint64_t key;
+        if (!frame_key(rr, x, y, x + (cx - px), y + (cy - py), bs, sr, key)) return;
+        uint32_t v;
+        if (!first && frame_lookup(f, key, v)) return;
+        const int i = base + k++;
+        f->ring_calls[i] = tcme_call{f->ref_pic[rr], f->cur_pic, x, y, x + (cx - px), y + (cy - py), bs, sr};
+        f->ring_res[i].status = TCME_E_HIP;
+        // the slot's previous call (if any) is no longer found: lookups check
+        // that the slot still holds the key they look for
+        f->ring_key[i] = key;
+        if (f->hcount * 4 > f->hkey.size() * 3) frame_table_rebuild(f);
+        frame_insert(f, key, kRingBit | static_cast<uint32_t>(i), true);
+    };
+    add(r, px, py, n, true);
+    const int x0 = px & ~63, y0 = py & ~63;
+    for (int q = 0; q < nr; ++q)
+        for (int bs = 64; bs >= 8; bs >>= 1)
+            for (int y = y0; y < y0 + 64 && y + bs <= f->h; y += bs)
+                for (int x = x0; x < x0 + 64 && x + bs <= f->w; x += bs)
+                    if (!(refs[q] == r && x == px && y == py && bs == n) && f->ref_pic[refs[q]] >= 0)
+                        add(refs[q], x, y, bs, false);
+    f->ring_pos += k;
+    hipStream_t s = f->st[f->next];
+    f->next ^= 1;
+    if (tcme::launch_list(g_drop.ctx->d_table, f->ring_calls_d + base, k, f->ring_res_d + base, s) !=
+        hipSuccess)
+        die("frame search launch");
+    const tcme_result res = ring_wait(f, base);
+    f->stats.misses++;
+    f->stats.batch_calls += k;
+    f->stats.miss_s += seconds_since(t0);
+    return res;
+}
+
+}  // namespace
+
+TCME_API tcme_mv tcme_frame_motion_est(tcme_frame *f, const uint8_t *ref, int ref_stride, int ref_w,
+                                       int ref_h, const uint8_t *orig, int orig_stride,
+                                       int center_x, int center_y, int blk_size,
+                                       int search_range, int32_t *best_sad_out)
+{
+    if (!f) {
+        fail(TCME_E_ARG, "null frame");
+        die("tcme_frame_motion_est");
+    }
+    std::unique_lock<std::mutex> g(f->mu);
+    f->stats.calls++;
+    // a call on the declared planes: which reference, and where the block is
+    int r = -1;
+    if (f->active && ref_w == f->w && ref_h == f->h && orig_stride == f->cur_stride)
+        for (int i = 0; i < f->nrefs && r < 0; ++i)
+            if (f->ref_host[i] == ref && f->ref_pic[i] >= 0) r = i;
+    int px = -1, py = -1;
+    if (r >= 0 && orig >= f->cur_host &&
+        orig < f->cur_host + static_cast<size_t>(f->cur_stride) * f->h) {
+        const size_t off = static_cast<size_t>(orig - f->cur_host);
+        px = static_cast<int>(off % f->cur_stride);
+        py = static_cast<int>(off / f->cur_stride);
+    }
+    uint64_t key;
+    if (r < 0 || px < 0 || px > f->w - blk_size || py > f->h - blk_size ||
+        !frame_key(r, px, py, center_x, center_y, blk_size, search_range, key)) {
+        // not a node of the declared frame: the per-call drop-in
+        f->stats.uncached++;
+        g.unlock();
+        return tc_motion_est(ref, ref_stride, ref_w, ref_h, orig, orig_stride, center_x, center_y,
+                             blk_size, search_range, best_sad_out);
+    }
+    if (f->spec_wait) {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (hipEventSynchronize(f->spec_done) != hipSuccess) die("frame-start search");
+        f->spec_wait = false;
+        f->stats.wait_s += seconds_since(t0);
+    }
+    tcme_result res;
+    uint32_t v;
+    if (frame_lookup(f, key, v) && !(v & kRingBit)) {
+        res = f->h_res[v];
+        f->stats.hits_spec++;
+    } else if (frame_lookup(f, key, v)) {
+        res = ring_wait(f, static_cast<int>(v & ~kRingBit));
+        f->stats.hits_batch++;
+    } else {
+        res = frame_miss(f, r, px, py, center_x, center_y, blk_size, search_range);
+    }
+    if (res.status != 0) {
+        fail(res.status, "frame search rejected a call (status %d)", res.status);
+        die("tcme_frame_motion_est");
+    }
+    if (best_sad_out) *best_sad_out = res.sad;
+    return tcme_mv{res.mv_x, res.mv_y};
+}
+
+TCME_API int tcme_frame_end(tcme_frame *f)
+{
+    if (!f) return fail(TCME_E_ARG, "null frame");
+    std::lock_guard<std::mutex> g(f->mu);
+    f->active = 0;
+    return TCME_OK;
+}
+
+TCME_API int tcme_frame_get_stats(tcme_frame *f, tcme_frame_stats *st)
+{
+    if (!f || !st) return fail(TCME_E_ARG, "null argument");
+    std::lock_guard<std::mutex> g(f->mu);
+    *st = f->stats;
+    return TCME_OK;
 }

@@ -30,7 +30,8 @@ EXPORTS = (
     "tcme_set_tiling", "tcme_get_tiling", "tcme_autotune", "tcme_set_timing",
     "tcme_last_timing", "tcme_pic_create_plane", "tcme_predict_luma", "tcme_predict_chroma",
     "tcme_global_mv", "tcme_sad", "tcme_graph_create", "tcme_graph_launch",
-    "tcme_graph_destroy",
+    "tcme_graph_destroy", "tcme_prepare", "tcme_frame_create", "tcme_frame_destroy",
+    "tcme_frame_begin", "tcme_frame_end", "tcme_frame_motion_est", "tcme_frame_get_stats",
 )
 
 CALL_DTYPE = np.dtype([("ref_pic", "<i4"), ("cur_pic", "<i4"), ("px", "<i4"), ("py", "<i4"),
