@@ -191,6 +191,19 @@ int  tcme_predict_chroma(tcme_ctx *ctx, const tcme_pred *preds, int n, uint8_t *
  * ordered on the device whatever their streams. */
 int  tcme_global_mv(tcme_ctx *ctx, const int32_t *pairs, int n, int32_t *out, void *stream);
 
+/* tcme_global_mv with no allocation at all: no counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The coarse
+ * SADs go to a fixed 16 MiB scratch array in device memory (one per GPU, held
+ * until the process ends) instead of the context's workspace; a call whose
+ * table would not fit (n x bands x 4,225 x 4 bytes, the bands set by the
+ * context's largest picture: about 152 KB a pair at 1920x1080, 574 KB at
+ * 3840x2160) fails with TCME_E_ARG.  Calls on one context are ordered on the
+ * device; calls on different contexts of one GPU that overlap in time race
+ * on the shared array. */
+int  tcme_global_mv_static(tcme_ctx *ctx, const int32_t *pairs, int n, int32_t *out,
+                           void *stream);
+
 /* A block SAD: the w x h block (1..64 each) of picture pic at (x,y) against
  * the block of picture other_pic at (ox,oy), or, with other_pic < 0, against
  * the caller's buffer at buf_off with row stride buf_stride.  Covers the SAD

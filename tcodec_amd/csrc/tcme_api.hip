@@ -104,6 +104,37 @@ struct tcme_ctx {
 
 namespace {
 
+// ---- scratch of the no-allocation entry point ------------------------------
+// tcme_global_mv_static keeps its coarse-SAD table in this fixed device array
+// instead of a context workspace.  Each GPU the library runs on holds its own
+// copy until the process ends.  In host code the array's name is only a
+// stand-in: its device address, per device, comes from hipGetSymbolAddress at
+// the first tcme_create on that device (scratch_setup) and only that address
+// is ever handed to a kernel.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+}  // namespace
+static __device__ __align__(256) int32_t scratch[kScratchBytes / sizeof(int32_t)];
+namespace {
+constexpr int kMaxDevices = 64;
+int32_t *g_scratch_ptr[kMaxDevices];
+std::once_flag g_scratch_once[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
+// Called with `device` current.
+int scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return TCME_OK;  // no scratch there
+    std::call_once(g_scratch_once[device], [device] {
+        void *p = nullptr;
+        g_scratch_err[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[device] = g_scratch_err[device] == hipSuccess ? static_cast<int32_t *>(p) : nullptr;
+    });
+    if (g_scratch_err[device] != hipSuccess)
+        return fail(TCME_E_HIP, "hipGetSymbolAddress(scratch): %s",
+                    hipGetErrorString(g_scratch_err[device]));
+    return TCME_OK;
+}
+
 int set_device(tcme_ctx *c)
 {
     HIP_TRY(hipSetDevice(c->device));
@@ -283,6 +314,10 @@ TCME_API int tcme_create(tcme_ctx **out, int device, int max_pics)
     if (!ok) {
         tcme_destroy(c);
         return fail(TCME_E_HIP, "context streams/events");
+    }
+    if (int e = scratch_setup(device)) {
+        tcme_destroy(c);
+        return e;
     }
     *out = c;
     return TCME_OK;
@@ -645,6 +680,37 @@ TCME_API int tcme_global_mv(tcme_ctx *c, const int32_t *pairs, int n, int32_t *o
     }
     HIP_TRY(tcme::launch_global_mv(c->d_table, c->max_pics, pairs, n, max_w, max_h, c->d_gmv, out,
                                    s));
+    HIP_TRY(hipEventRecord(c->gmv_done, s));
+    c->gmv_pending = true;
+    return TCME_OK;
+}
+
+// tcme_global_mv without allocation: the coarse-SAD table lives in the fixed
+// per-device scratch array.  Every entry the walk reads is written by the
+// table kernel first, so nothing needs clearing between calls.
+TCME_API int tcme_global_mv_static(tcme_ctx *c, const int32_t *pairs, int n, int32_t *out,
+                                   void *stream)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    if (n < 0 || (n > 0 && (!pairs || !out))) return fail(TCME_E_ARG, "bad pair list");
+    if (n > 65535) return fail(TCME_E_ARG, "%d pairs in one call (at most 65535)", n);
+    if (n == 0) return TCME_OK;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device])
+        return fail(TCME_E_HIP, "no scratch address for device %d", c->device);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int max_w = 1, max_h = 1;
+    for (const Pic &p : c->pics)
+        if (p.used) { max_w = std::max(max_w, p.w); max_h = std::max(max_h, p.h); }
+    const size_t need = tcme::gmv_part_bytes(n, max_w, max_h);
+    if (need > kScratchBytes)
+        return fail(TCME_E_ARG, "%d pairs of pictures up to %dx%d need %zu bytes of scratch "
+                    "(at most %zu): use fewer pairs per call, or tcme_global_mv",
+                    n, max_w, max_h, need, kScratchBytes);
+    if (c->gmv_pending) HIP_TRY(hipStreamWaitEvent(s, c->gmv_done, 0));
+    HIP_TRY(tcme::launch_global_mv(c->d_table, c->max_pics, pairs, n, max_w, max_h,
+                                   g_scratch_ptr[c->device], out, s));
     HIP_TRY(hipEventRecord(c->gmv_done, s));
     c->gmv_pending = true;
     return TCME_OK;

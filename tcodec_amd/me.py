@@ -32,6 +32,7 @@ EXPORTS = (
     "tcme_global_mv", "tcme_sad", "tcme_graph_create", "tcme_graph_launch",
     "tcme_graph_destroy", "tcme_prepare", "tcme_frame_create", "tcme_frame_destroy",
     "tcme_frame_begin", "tcme_frame_end", "tcme_frame_motion_est", "tcme_frame_get_stats",
+    "tcme_global_mv_static",
 )
 
 CALL_DTYPE = np.dtype([("ref_pic", "<i4"), ("cur_pic", "<i4"), ("px", "<i4"), ("py", "<i4"),
@@ -126,6 +127,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tcme_set_timing.argtypes = [C.c_void_p, C.c_int]
     L.tcme_pic_create_plane.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.tcme_global_mv.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    L.tcme_global_mv_static.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                        C.c_void_p]
     L.tcme_sad.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
                            C.c_void_p]
     for f in (L.tcme_predict_luma, L.tcme_predict_chroma):
