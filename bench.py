@@ -506,6 +506,63 @@ def mc_leg(eng, frames, d_res, scene=None, pics=None, reps: int = 5):
     return out
 
 
+def chain_leg(eng, pics, reps: int = 3):
+    """The ME-only predictor chain (tcme_search_chain) on the headline's 8
+    coded 1080p pictures: each CTU's searches centred at qt_mvp of its own
+    grid, in the reference's decision order, one workgroup per CTU.  P
+    pictures search dpb[0] and dpb[1] (the reference's multi-reference P),
+    B pictures the nearest earlier and later pictures (its forward and
+    backward references); lambda 4.  One launch for all 8 pictures, and one
+    launch per picture."""
+    import torch
+    from tcodec_amd import me, workload
+    order = workload.gop4_coding_order(DISPLAY_FRAMES)
+    rec = np.zeros(len(TIMED_EMITS), me.CHAIN_FRAME_DTYPE)
+    for i, e in enumerate(TIMED_EMITS):
+        poc, kind, dpb = order[e]
+        if kind == "P":
+            refs, sr = dpb[:2], workload.SR_P
+        else:
+            refs = [max(p for p in dpb if p < poc), min(p for p in dpb if p > poc)]
+            sr = workload.SR_B
+        rec[i]["cur_pic"], rec[i]["nrefs"] = pics[poc], 2
+        rec[i]["ref_pic"][:2] = [pics[p] for p in refs]
+        rec[i]["sr"], rec[i]["lambda"] = sr, 4
+    nct = eng.chain_ctus(int(rec[0]["cur_pic"]))
+    total = nct * len(rec)
+    d_calls = torch.empty(total * me.CHAIN_MAX_CALLS * me.CHAIN_CALL_DTYPE.itemsize,
+                          dtype=torch.uint8, device="cuda")
+    d_counts = torch.zeros(total, dtype=torch.int32, device="cuda")
+    s = torch.cuda.current_stream()
+
+    def all_in_one():
+        eng.chain_device(rec, d_calls.data_ptr(), d_counts.data_ptr(), total, s.cuda_stream)
+
+    def per_picture():
+        for i in range(len(rec)):
+            off = i * nct
+            eng.chain_device(rec[i:i + 1], d_calls.data_ptr() + off * me.CHAIN_MAX_CALLS * 40,
+                             d_counts.data_ptr() + off * 4, nct, s.cuda_stream)
+
+    out = {}
+    for name, fn in (("one_launch", all_in_one), ("launch_per_picture", per_picture)):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"ms_per_8_pictures": round(ms, 3),
+                     "frames_per_s": round(len(rec) / ms * 1e3, 1)}
+    calls = int(d_counts.sum().item())
+    res = {"kernel": "me_chain_kernel", "pictures": len(rec), "ctus_per_picture": nct,
+           "refs": 2, "lambda": 4, "searches_per_picture": calls // len(rec), **out}
+    log(f"chain: {res}")
+    return res
+
+
 def aux_leg(eng, frames, pics, reps: int = 3):
     """Global-motion estimates for every timed picture against its nearest
     reference, the co-located 8x8 and 64x64 stability SADs of one picture
@@ -741,6 +798,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(scene, pics, frames, d_res[0].cpu().numpy())
     mc = mc_leg(eng, frames, d_res, scene, pics)
     aux = aux_leg(eng, frames, pics)
+    chain = chain_leg(eng, pics)
     try:
         encoder = None if args.no_cpu else encoder_leg()
     except (OSError, subprocess.SubprocessError) as e:
@@ -806,6 +864,7 @@ def main():
         "host_api": host_api,
         "mc": mc,
         "aux": aux,
+        "mvp_chain": chain,
         "encoder": encoder,
         "batched": batched,
         "served": served,

@@ -155,6 +155,46 @@ int  tcme_graph_create(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result
 int  tcme_graph_launch(tcme_graph *g, void *stream);
 void tcme_graph_destroy(tcme_graph *g);
 
+/* ---- ME-only predictor chain ------------------------------------------- */
+
+/* The reference encoder centres every search at the min-magnitude predictor
+ * qt_mvp (src/encoder.c:405-427) of a per-CTU 8x8 grid (reset per CTU,
+ * :1425-1428) that its RDO fills with each leaf's final displacement
+ * (:1279-1288), visiting nodes in qt_split's decision order (:1292-1397).
+ * The chain runs that dependency on the device without the RDO: one
+ * workgroup per CTU walks the nodes in that order, centres each leaf's
+ * searches (one per reference, in order) at qt_mvp of the grid, keeps the
+ * cheapest J = SAD + lambda * (se_bits(mvd.x) + se_bits(mvd.y)) (first
+ * reference on ties; se_bits as src/bitstream.c:100-112; mvd = mv - (mvp +
+ * 4 pos)), fills the leaf's cells with the chosen displacement, and keeps the
+ * four children when their J sum is not above the parent's.  Every search is
+ * tc_motion_est's own; the decisions are the chain's (ME-only), so the calls
+ * are those an encoder with this decision rule would make. */
+typedef struct tcme_chain_frame {
+    int32_t cur_pic, nrefs;   /* 1..4 references, searched in this order */
+    int32_t ref_pic[4];
+    int32_t sr, lambda;       /* search range; lambda >= 0 */
+} tcme_chain_frame;
+
+/* One search of the chain: reference index, node, centre, block size,
+ * result. */
+typedef struct tcme_chain_call {
+    int32_t ref, px, py, cx, cy, blk;
+    int32_t mv_x, mv_y, sad, reserved;
+} tcme_chain_call;
+
+/* Records per CTU in the output: at most 169 leaves (1 + 4 + 4 x (1 + 4 + 4 x (1 + 4 + 4))) x 4 references. */
+#define TCME_CHAIN_MAX_CALLS 676
+
+/* Runs the chain over n frames (host array, 1 <= n <= 16; every reference the
+ * size of its current picture).  The frames' CTUs are numbered frame after
+ * frame, raster order, ceil(w/64) x ceil(h/64) per frame; CTU k's searches go
+ * to calls[k * TCME_CHAIN_MAX_CALLS ...] in the order they are made and their
+ * number to counts[k].  calls and counts are device arrays with room for
+ * ctu_cap CTUs.  Asynchronous on stream. */
+int  tcme_search_chain(tcme_ctx *ctx, const tcme_chain_frame *frames, int n,
+                       tcme_chain_call *calls, int32_t *counts, int64_t ctu_cap, void *stream);
+
 /* ---- motion compensation (batched) -------------------------------------- */
 
 /* One predicted block: picture id, absolute motion vector (quarter sample

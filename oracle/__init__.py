@@ -94,6 +94,12 @@ class Oracle:
         L.orc_mvp.argtypes = [C.c_int, C.c_int, i16p, i16p, u8p, i32p, i32p]
         L.orc_global_mv.restype = None
         L.orc_global_mv.argtypes = [u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int, i32p, i32p]
+        L.orc_se_bits.restype = C.c_int
+        L.orc_se_bits.argtypes = [C.c_int32]
+        L.orc_chain_frame.restype = C.c_int
+        L.orc_chain_frame.argtypes = [C.POINTER(u8p), C.c_int, C.c_int, u8p, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, i32p,
+                                      C.c_int]
 
     # -- convenience wrappers over numpy planes (C-contiguous uint8 2-D) --
     def motion_est(self, ref, cur, px, py, cx, cy, n, sr, stats=False):
@@ -149,6 +155,25 @@ class Oracle:
                          C.byref(oy))
         return ox.value, oy.value
 
+    CHAIN_ROW = 10
+
+    def chain_frame(self, refs, cur, sr, lam, mvp=None, me=None):
+        """The ME-only predictor chain over one frame: int32 [N, 10] rows
+        (ctu, ref, px, py, cx, cy, n, mv_x, mv_y, sad) in call order.  mvp /
+        me: None (the restatement) or C function pointers (e.g. the
+        reference's own, from Reference.fn_pointers())."""
+        h, w = cur.shape
+        refs = [np.ascontiguousarray(r, np.uint8) for r in refs]
+        assert all(r.shape == cur.shape and r.strides == refs[0].strides for r in refs)
+        arr = (u8p * len(refs))(*[ptr(r) for r in refs])
+        cols, rows = (w + 63) // 64, (h + 63) // 64
+        cap = cols * rows * 169 * len(refs)
+        out = np.zeros((cap, self.CHAIN_ROW), np.int32)
+        n = self.lib.orc_chain_frame(arr, len(refs), refs[0].strides[0], ptr(cur), cur.strides[0],
+                                     w, h, sr, lam, mvp, me, ptr(out, i32p), cap)
+        assert n <= cap
+        return out[:n].copy()
+
     def global_mv(self, cur, ref):
         h, w = ref.shape
         gx, gy = C.c_int32(), C.c_int32()
@@ -181,9 +206,16 @@ class Reference:
         L.refx_qt_mvp.argtypes = [C.c_int, C.c_int, i16p, i16p, u8p, i32p, i32p]
         L.refx_global_mv.restype = C.c_int
         L.refx_global_mv.argtypes = [u8p, C.c_int, u8p, C.c_int, C.c_int, C.c_int, i32p, i32p]
+        L.refx_qt_mvp_v.restype = None
         L.refx_motion_est_batch.restype = C.c_int
         L.refx_motion_est_batch.argtypes = [u8p, C.c_int, C.c_int, C.c_int, u8p, C.c_int, i32p,
                                             C.c_int, i32p]
+
+    def fn_pointers(self):
+        """(qt_mvp, tc_motion_est) of the reference as C function pointers, for
+        Oracle.chain_frame."""
+        return (C.cast(self.lib.refx_qt_mvp_v, C.c_void_p).value,
+                C.cast(self.lib.tc_motion_est, C.c_void_p).value)
 
     def motion_est(self, ref, cur, px, py, cx, cy, n, sr, w=None, h=None):
         hh, ww = ref.shape

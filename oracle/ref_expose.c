@@ -7,6 +7,7 @@
  * restatement of them against the real code:
  *
  *   refx_qt_mvp        -> qt_mvp              (src/encoder.c:405-427)
+ *   refx_qt_mvp_v      -> the same, void (a function pointer for the oracle)
  *   refx_global_mv     -> estimate_global_mv  (src/encoder.c:171-202)
  *
  * and one loop over the reference's public tc_motion_est (src/motion.c:384),
@@ -31,6 +32,14 @@ int refx_qt_mvp(int cx, int cy, const int16_t *dx, const int16_t *dy,
     *out_x = m.x;
     *out_y = m.y;
     return 0;
+}
+
+/* The same with orc_mvp_fn's exact type (void), so the oracle's chain can
+ * call the reference's predictor through a function pointer. */
+void refx_qt_mvp_v(int cx, int cy, const int16_t *dx, const int16_t *dy, const uint8_t *intra,
+                   int *out_x, int *out_y)
+{
+    refx_qt_mvp(cx, cy, dx, dy, intra, out_x, out_y);
 }
 
 int refx_global_mv(const uint8_t *cur, int cur_stride,

@@ -20,6 +20,8 @@
  */
 #include "tcme_oracle.h"
 #include <stddef.h>
+#include <stdint.h>
+#include <string.h>
 
 static inline int clip255(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
 static inline int iabs(int v) { return v < 0 ? -v : v; }
@@ -361,4 +363,158 @@ void orc_global_mv(const uint8_t *cur, int cstride, const uint8_t *ref, int rstr
     }
     *gx = ox * 4;
     *gy = oy * 4;
+}
+
+/* ---- ME-only predictor chain -------------------------------------------
+ * The reference encoder centres each search at qt_mvp over a per-CTU 8x8
+ * grid (src/encoder.c:405-427; the grid is reset to "unavailable" per CTU at
+ * :1425-1428) that its RDO fills with each leaf's final displacement
+ * (:1279-1288), visiting nodes in qt_split's decision order (:1292-1397: the
+ * parent as a leaf, the four children as leaves, then, when the children win
+ * (:1388, child <= parent), each child again as a subtree; nodes not wholly
+ * in the frame are split without a search, :1299-1306).  The chain keeps that
+ * order, that grid and that predictor and replaces the RDO with an ME-only
+ * rule: a leaf searches every reference at ((mvp + 4 pos) >> 2) and keeps the
+ * cheapest J = SAD + lambda * (se_bits(mvd.x) + se_bits(mvd.y)) (the first
+ * reference on ties), mvd = mv - (mvp + 4 pos), se_bits as
+ * src/bitstream.c:100-112; its cells take the chosen displacement mv - 4 pos
+ * (as :1283-1285 stores mvp + mvd).  The predictor and the search are
+ * function pointers so the tests can run the chain on the reference's own
+ * qt_mvp and tc_motion_est. */
+
+int orc_se_bits(int32_t v)
+{
+    uint32_t m = v > 0 ? (uint32_t)(2 * (int64_t)v - 1) : v < 0 ? (uint32_t)(-2 * (int64_t)v) : 0u;
+    uint64_t code = (uint64_t)m + 1;
+    int bits = 0;
+    while (code) { bits++; code >>= 1; }
+    return 2 * bits - 1;
+}
+
+static orc_mv chain_me_default(const uint8_t *ref, int stride, int w, int h, const uint8_t *orig,
+                               int ostride, int cx, int cy, int n, int sr, int32_t *sad)
+{
+    return orc_motion_est(ref, stride, w, h, orig, ostride, cx, cy, n, sr, sad, NULL);
+}
+
+typedef struct {
+    const uint8_t *const *refs;
+    int nrefs, rstride, w, h, sr, lambda;
+    const uint8_t *cur;
+    int cstride;
+    orc_mvp_fn mvp;
+    orc_me_fn me;
+    int16_t dx[64], dy[64];
+    uint8_t intra[64];
+    int ctu, ctu_x, ctu_y;
+    int32_t *out;
+    int n, cap;
+} chain_ctx;
+
+static int64_t chain_leaf(chain_ctx *c, int depth, int cx, int cy)
+{
+    const int cu = 64 >> depth, px = c->ctu_x + 8 * cx, py = c->ctu_y + 8 * cy;
+    int mx, my;
+    c->mvp(cx, cy, c->dx, c->dy, c->intra, &mx, &my);
+    const int ccx = (mx + 4 * px) >> 2, ccy = (my + 4 * py) >> 2;
+    int64_t best = INT64_MAX;
+    int bdx = 0, bdy = 0;
+    for (int r = 0; r < c->nrefs; r++) {
+        int32_t sad = 0;
+        orc_mv mv = c->me(c->refs[r], c->rstride, c->w, c->h,
+                          c->cur + (ptrdiff_t)py * c->cstride + px, c->cstride, ccx, ccy, cu,
+                          c->sr, &sad);
+        const int64_t j = (int64_t)sad + (int64_t)c->lambda *
+                          (orc_se_bits(mv.x - (mx + 4 * px)) + orc_se_bits(mv.y - (my + 4 * py)));
+        if (c->n < c->cap) {
+            int32_t *row = c->out + (ptrdiff_t)c->n * ORC_CHAIN_ROW;
+            row[0] = c->ctu; row[1] = r; row[2] = px; row[3] = py; row[4] = ccx; row[5] = ccy;
+            row[6] = cu; row[7] = mv.x; row[8] = mv.y; row[9] = sad;
+        }
+        c->n++;
+        if (j < best) { best = j; bdx = mv.x - 4 * px; bdy = mv.y - 4 * py; }
+    }
+    const int cs = cu / 8;
+    for (int gy = 0; gy < cs; gy++)
+        for (int gx = 0; gx < cs; gx++) {
+            const int i = (cy + gy) * 8 + cx + gx;
+            c->dx[i] = (int16_t)bdx;
+            c->dy[i] = (int16_t)bdy;
+            c->intra[i] = 0;
+        }
+    return best;
+}
+
+static void chain_save(const chain_ctx *c, int cx, int cy, int cs, int16_t *dx, int16_t *dy,
+                       uint8_t *in)
+{
+    for (int gy = 0; gy < cs; gy++)
+        for (int gx = 0; gx < cs; gx++) {
+            const int i = (cy + gy) * 8 + cx + gx, k = gy * cs + gx;
+            dx[k] = c->dx[i]; dy[k] = c->dy[i]; in[k] = c->intra[i];
+        }
+}
+
+static void chain_load(chain_ctx *c, int cx, int cy, int cs, const int16_t *dx, const int16_t *dy,
+                       const uint8_t *in)
+{
+    for (int gy = 0; gy < cs; gy++)
+        for (int gx = 0; gx < cs; gx++) {
+            const int i = (cy + gy) * 8 + cx + gx, k = gy * cs + gx;
+            c->dx[i] = dx[k]; c->dy[i] = dy[k]; c->intra[i] = in[k];
+        }
+}
+
+static void chain_split(chain_ctx *c, int depth, int cx, int cy)
+{
+    const int cu = 64 >> depth, px = c->ctu_x + 8 * cx, py = c->ctu_y + 8 * cy;
+    const int in_frame = px + cu <= c->w && py + cu <= c->h;
+    const int half = cu / 16;  /* child offset in cells */
+    if (!in_frame && cu > 8) {
+        for (int q = 0; q < 4; q++) chain_split(c, depth + 1, cx + (q & 1) * half, cy + (q >> 1) * half);
+        return;
+    }
+    if (!in_frame) return;
+    if (cu == 8) {
+        chain_leaf(c, depth, cx, cy);
+        return;
+    }
+    const int cs = cu / 8;
+    int16_t pdx[64], pdy[64], qdx[64], qdy[64];
+    uint8_t pin[64], qin[64];
+    chain_save(c, cx, cy, cs, pdx, pdy, pin);
+    const int64_t parent = chain_leaf(c, depth, cx, cy);
+    chain_save(c, cx, cy, cs, qdx, qdy, qin);
+    chain_load(c, cx, cy, cs, pdx, pdy, pin);
+    int64_t child = 0;
+    for (int q = 0; q < 4; q++) child += chain_leaf(c, depth + 1, cx + (q & 1) * half, cy + (q >> 1) * half);
+    if (child <= parent) {
+        chain_load(c, cx, cy, cs, pdx, pdy, pin);
+        for (int q = 0; q < 4; q++) chain_split(c, depth + 1, cx + (q & 1) * half, cy + (q >> 1) * half);
+    } else {
+        chain_load(c, cx, cy, cs, qdx, qdy, qin);
+    }
+}
+
+int orc_chain_frame(const uint8_t *const *refs, int nrefs, int rstride, const uint8_t *cur,
+                    int cstride, int w, int h, int sr, int lambda, orc_mvp_fn mvp, orc_me_fn me,
+                    int32_t *out, int cap)
+{
+    chain_ctx c;
+    memset(&c, 0, sizeof c);
+    c.refs = refs; c.nrefs = nrefs; c.rstride = rstride; c.w = w; c.h = h; c.sr = sr;
+    c.lambda = lambda; c.cur = cur; c.cstride = cstride;
+    c.mvp = mvp ? mvp : orc_mvp;
+    c.me = me ? me : chain_me_default;
+    c.out = out; c.cap = cap;
+    const int cols = (w + 63) / 64, rows = (h + 63) / 64;
+    for (int ry = 0; ry < rows; ry++)
+        for (int rx = 0; rx < cols; rx++) {
+            c.ctu = ry * cols + rx;
+            c.ctu_x = rx * 64;
+            c.ctu_y = ry * 64;
+            for (int i = 0; i < 64; i++) { c.dx[i] = 0; c.dy[i] = 0; c.intra[i] = 1; }
+            chain_split(&c, 0, 0, 0);
+        }
+    return c.n;
 }

@@ -62,6 +62,21 @@ void    orc_mvp(int cx, int cy, const int16_t *dx, const int16_t *dy,
 void    orc_global_mv(const uint8_t *cur, int cstride, const uint8_t *ref, int rstride,
                       int w, int h, int *gx, int *gy);
 
+/* ME-only predictor chain over one frame (restated rules in tcme_oracle.c):
+ * rows of ORC_CHAIN_ROW int32 {ctu, ref, px, py, cx, cy, n, mv_x, mv_y, sad},
+ * CTUs in raster order, each CTU's calls in the order they are made.  mvp and
+ * me may be NULL (orc_mvp, orc_motion_est) or the reference's own functions.
+ * Returns the call count (rows past cap are counted, not written). */
+#define ORC_CHAIN_ROW 10
+typedef void (*orc_mvp_fn)(int cx, int cy, const int16_t *dx, const int16_t *dy,
+                           const uint8_t *intra, int *out_x, int *out_y);
+typedef orc_mv (*orc_me_fn)(const uint8_t *ref, int stride, int w, int h, const uint8_t *orig,
+                            int ostride, int cx, int cy, int n, int sr, int32_t *sad);
+int     orc_se_bits(int32_t v);
+int     orc_chain_frame(const uint8_t *const *refs, int nrefs, int rstride, const uint8_t *cur,
+                        int cstride, int w, int h, int sr, int lambda, orc_mvp_fn mvp,
+                        orc_me_fn me, int32_t *out, int cap);
+
 #ifdef __cplusplus
 }
 #endif

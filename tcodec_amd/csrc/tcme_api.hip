@@ -728,6 +728,49 @@ TCME_API int tcme_sad(tcme_ctx *c, const tcme_sad_desc *descs, int n, const uint
     return TCME_OK;
 }
 
+TCME_API int tcme_search_chain(tcme_ctx *c, const tcme_chain_frame *frames, int n,
+                              tcme_chain_call *calls, int32_t *counts, int64_t ctu_cap,
+                              void *stream)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    if (n < 1 || n > tcme::kChainFrames || !frames || !calls || !counts)
+        return fail(TCME_E_ARG, "bad chain frame list (1..%d frames)", tcme::kChainFrames);
+    tcme::ChainJobHost jobs[tcme::kChainFrames];
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const tcme_chain_frame &f = frames[i];
+        if (int e = check_pic(c, f.cur_pic)) return e;
+        const Pic &cp = c->pics[f.cur_pic];
+        if (f.nrefs < 1 || f.nrefs > 4 || f.sr < 0 || f.sr > 16383 || f.lambda < 0 ||
+            f.lambda > (1 << 24))
+            return fail(TCME_E_ARG, "chain frame %d: nrefs %d, sr %d, lambda %d", i, f.nrefs,
+                        f.sr, f.lambda);
+        for (int r = 0; r < f.nrefs; ++r) {
+            if (int e = check_pic(c, f.ref_pic[r])) return e;
+            const Pic &rp = c->pics[f.ref_pic[r]];
+            if (!rp.d_an || rp.w != cp.w || rp.h != cp.h)
+                return fail(TCME_E_PIC, "chain frame %d: reference %d is not a %dx%d reference "
+                            "picture", i, r, cp.w, cp.h);
+        }
+        tcme::ChainJobHost &j = jobs[i];
+        j.cur_pic = f.cur_pic;
+        j.nrefs = f.nrefs;
+        for (int r = 0; r < 4; ++r) j.ref_pic[r] = r < f.nrefs ? f.ref_pic[r] : f.ref_pic[0];
+        j.sr = f.sr;
+        j.lambda = f.lambda;
+        j.first = static_cast<int>(total);
+        j.cols = (cp.w + 63) / 64;
+        total += static_cast<int64_t>(j.cols) * ((cp.h + 63) / 64);
+    }
+    if (total > ctu_cap) return fail(TCME_E_ARG, "%lld CTUs, room for %lld", (long long)total,
+                                     (long long)ctu_cap);
+    if (total > (1 << 30)) return fail(TCME_E_ARG, "too many CTUs");
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    HIP_TRY(tcme::launch_chain(c->d_table, jobs, n, static_cast<int>(total), calls, counts,
+                               static_cast<hipStream_t>(stream)));
+    return TCME_OK;
+}
+
 TCME_API int tcme_set_timing(tcme_ctx *c, int on)
 {
     if (!c) return fail(TCME_E_ARG, "null context");

@@ -68,6 +68,18 @@ hipError_t launch_anchors(const PicDev &p, hipStream_t s);
 // followed by at least kTailSlack readable bytes.
 hipError_t launch_dropin(const PicDev *table, const tcme_call &c, const uint8_t *blk,
                          tcme_result *res, hipStream_t s);
+// The ME-only predictor chain: up to kChainFrames frames per launch, one
+// workgroup per CTU (frames' CTUs numbered from `first`, raster order, `cols`
+// CTUs per row), kChainMaxCalls records per CTU in `calls`.  Jobs are
+// validated by the host.
+constexpr int kChainFrames = 16;
+constexpr int kChainMaxCalls = 676;
+struct ChainJobHost {
+    int cur_pic, nrefs, ref_pic[4], sr, lambda, first, cols;
+};
+hipError_t launch_chain(const PicDev *table, const ChainJobHost *jobs, int njobs, int total_ctus,
+                        tcme_chain_call *calls, int32_t *counts, hipStream_t s);
+
 // n validated calls on resident pictures (calls and res device-visible,
 // normally mapped host memory), one workgroup per call, any supported size;
 // each result's status is written last, after a system-scope fence.

@@ -838,6 +838,192 @@ __global__ __launch_bounds__(256) void me_list_kernel(const PicDev *__restrict__
     }
 }
 
+// ---- ME-only predictor chain ------------------------------------------------
+// One workgroup of 4 wavefronts per CTU walks the CTU's quadtree in the
+// reference encoder's decision order (src/encoder.c:1292-1397: a node as a
+// leaf, its four children as leaves, and, when the children's cost is not
+// above the parent's, each child again as a subtree; nodes not wholly in the
+// frame are split without a search).  Every leaf is centred at qt_mvp
+// (src/encoder.c:405-427) of the CTU's 8x8 displacement grid, reset to
+// "unavailable" per CTU (:1425-1428); it searches each reference in turn and
+// keeps the cheapest J = SAD + lambda * (se_bits(mvd.x) + se_bits(mvd.y))
+// (first on ties), and its cells take the chosen displacement.  The control
+// flow is uniform: every thread holds the same walk state; the grid lives in
+// LDS.  8x8 and 16x16 searches run on the first wavefront, larger ones on all
+// four.
+struct ChainJob {
+    int32_t cur_pic, nrefs, ref_pic[4], sr, lambda, first, cols;
+};
+struct ChainArgs {
+    int32_t njobs, pad;
+    ChainJob job[kChainFrames];
+};
+
+struct ChainLds {
+    union {
+        Shared<8, 1> s8;
+        Shared<16, 1> s16;
+        Shared<32, 4> s32;
+        Shared<64, 4> s64;
+    } sh;
+    tcme_result r;
+    int16_t dx[64], dy[64];
+    uint8_t un[64];                          // 1: unavailable (intra)
+    int16_t sdx[2][3][64], sdy[2][3][64];    // saved regions per depth: pre, parent
+    uint8_t sun[2][3][64];
+};
+
+struct ChainWalk {
+    const PicDev *table;
+    PicDev C;
+    ChainJob j;
+    int ctu_x, ctu_y, w, h;
+    tcme_chain_call *out;
+    int nout;
+};
+
+__device__ __forceinline__ int se_bits(int v)
+{
+    const uint32_t m = v > 0 ? 2u * static_cast<uint32_t>(v) - 1u : (v < 0 ? 2u * static_cast<uint32_t>(-v) : 0u);
+    return 2 * (32 - __builtin_clz(m + 1u)) - 1;  // m + 1 >= 1
+}
+
+// qt_mvp (src/encoder.c:405-427) of cell (cx, cy) from the LDS grid.
+__device__ __forceinline__ void chain_mvp(const ChainLds &L, int cx, int cy, int &mx, int &my)
+{
+    int ax = 0, ay = 0, bx = 0, by = 0, qx = 0, qy = 0;
+    bool ha = false, hb = false, hc = false;
+    if (cx > 0 && !L.un[cy * 8 + cx - 1]) { ax = L.dx[cy * 8 + cx - 1]; ay = L.dy[cy * 8 + cx - 1]; ha = true; }
+    if (cy > 0 && !L.un[(cy - 1) * 8 + cx]) { bx = L.dx[(cy - 1) * 8 + cx]; by = L.dy[(cy - 1) * 8 + cx]; hb = true; }
+    if (cy > 0 && cx < 7 && !L.un[(cy - 1) * 8 + cx + 1]) {
+        qx = L.dx[(cy - 1) * 8 + cx + 1]; qy = L.dy[(cy - 1) * 8 + cx + 1]; hc = true;
+    }
+    if (!ha && !hb && !hc) { mx = 0; my = 0; return; }
+    if (!ha) { ax = hb ? bx : qx; ay = hb ? by : qy; }
+    if (!hb) { bx = ha ? ax : qx; by = ha ? ay : qy; }
+    if (!hc) { qx = ha ? ax : bx; qy = ha ? ay : by; }
+    const int64_t ma = static_cast<int64_t>(ax) * ax + static_cast<int64_t>(ay) * ay;
+    const int64_t mb = static_cast<int64_t>(bx) * bx + static_cast<int64_t>(by) * by;
+    const int64_t mc = static_cast<int64_t>(qx) * qx + static_cast<int64_t>(qy) * qy;
+    if (mb < ma && mb <= mc) { mx = bx; my = by; }
+    else if (mc < ma && mc < mb) { mx = qx; my = qy; }
+    else { mx = ax; my = ay; }
+}
+
+// Copies the cs x cs region at (cx, cy) of the grid to or from save slot
+// (k, d); one cell per thread of the first 64.
+__device__ __forceinline__ void chain_copy(ChainLds &L, int cx, int cy, int cs, int k, int d, bool save)
+{
+    const int t = static_cast<int>(threadIdx.x);
+    if (t < cs * cs) {
+        const int i = (cy + t / cs) * 8 + cx + t % cs;
+        if (save) { L.sdx[k][d][t] = L.dx[i]; L.sdy[k][d][t] = L.dy[i]; L.sun[k][d][t] = L.un[i]; }
+        else { L.dx[i] = L.sdx[k][d][t]; L.dy[i] = L.sdy[k][d][t]; L.un[i] = L.sun[k][d][t]; }
+    }
+    __syncthreads();
+}
+
+template <int D>
+__device__ __forceinline__ int64_t chain_leaf(ChainLds &L, ChainWalk &W, int cx, int cy)
+{
+    constexpr int N = 64 >> D;
+    const int px = W.ctu_x + 8 * cx, py = W.ctu_y + 8 * cy;
+    int mx, my;
+    chain_mvp(L, cx, cy, mx, my);
+    const int ccx = (mx + 4 * px) >> 2, ccy = (my + 4 * py) >> 2;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    int64_t best = INT64_MAX;
+    int bdx = 0, bdy = 0;
+    for (int r = 0; r < W.j.nrefs; ++r) {
+        const PicDev R = W.table[W.j.ref_pic[r]];
+        const tcme_call c{W.j.ref_pic[r], W.j.cur_pic, px, py, ccx, ccy, N, W.j.sr};
+        if constexpr (N <= 16) {
+            if (wave == 0) {
+                if constexpr (N == 8) search<8, 1>(R, W.C, c, &L.r, nullptr, L.sh.s8.win, &L.sh.s8.red[0][0][0]);
+                else search<16, 1>(R, W.C, c, &L.r, nullptr, L.sh.s16.win, &L.sh.s16.red[0][0][0]);
+            }
+        } else if constexpr (N == 32) {
+            search<32, 4>(R, W.C, c, &L.r, nullptr, L.sh.s32.win, &L.sh.s32.red[0][0][0]);
+        } else {
+            search<64, 4>(R, W.C, c, &L.r, nullptr, L.sh.s64.win, &L.sh.s64.red[0][0][0]);
+        }
+        __syncthreads();
+        const int mvx = L.r.mv_x, mvy = L.r.mv_y, sad = L.r.sad;
+        __syncthreads();  // r and the search's LDS are reused by the next search
+        const int64_t jc = static_cast<int64_t>(sad) +
+                           static_cast<int64_t>(W.j.lambda) * (se_bits(mvx - (mx + 4 * px)) + se_bits(mvy - (my + 4 * py)));
+        if (threadIdx.x == 0 && W.nout < kChainMaxCalls)
+            W.out[W.nout] = tcme_chain_call{r, px, py, ccx, ccy, N, mvx, mvy, sad, 0};
+        ++W.nout;
+        if (jc < best) { best = jc; bdx = mvx - 4 * px; bdy = mvy - 4 * py; }
+    }
+    constexpr int cs = N / 8;
+    const int t = static_cast<int>(threadIdx.x);
+    if (t < cs * cs) {
+        const int i = (cy + t / cs) * 8 + cx + t % cs;
+        L.dx[i] = static_cast<int16_t>(bdx);
+        L.dy[i] = static_cast<int16_t>(bdy);
+        L.un[i] = 0;
+    }
+    __syncthreads();
+    return best;
+}
+
+template <int D>
+__device__ __forceinline__ void chain_split(ChainLds &L, ChainWalk &W, int cx, int cy)
+{
+    constexpr int N = 64 >> D, half = N / 16;
+    const int px = W.ctu_x + 8 * cx, py = W.ctu_y + 8 * cy;
+    const bool in_frame = px + N <= W.w && py + N <= W.h;
+    if constexpr (D == 3) {
+        if (in_frame) chain_leaf<3>(L, W, cx, cy);
+    } else {
+        if (!in_frame) {
+            for (int q = 0; q < 4; ++q) chain_split<D + 1>(L, W, cx + (q & 1) * half, cy + (q >> 1) * half);
+            return;
+        }
+        constexpr int cs = N / 8;
+        chain_copy(L, cx, cy, cs, 0, D, true);
+        const int64_t parent = chain_leaf<D>(L, W, cx, cy);
+        chain_copy(L, cx, cy, cs, 1, D, true);
+        chain_copy(L, cx, cy, cs, 0, D, false);
+        int64_t child = 0;
+        for (int q = 0; q < 4; ++q) child += chain_leaf<D + 1>(L, W, cx + (q & 1) * half, cy + (q >> 1) * half);
+        if (child <= parent) {
+            chain_copy(L, cx, cy, cs, 0, D, false);
+            for (int q = 0; q < 4; ++q) chain_split<D + 1>(L, W, cx + (q & 1) * half, cy + (q >> 1) * half);
+        } else {
+            chain_copy(L, cx, cy, cs, 1, D, false);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void me_chain_kernel(const PicDev *__restrict__ table, ChainArgs a,
+                                                       tcme_chain_call *__restrict__ calls,
+                                                       int32_t *__restrict__ counts)
+{
+    __shared__ ChainLds L;
+    const int b = static_cast<int>(blockIdx.x);
+    int k = 0;
+    while (k + 1 < a.njobs && b >= a.job[k + 1].first) ++k;
+    ChainWalk W;
+    W.table = table;
+    W.j = a.job[k];
+    W.C = table[W.j.cur_pic];
+    W.w = W.C.w;
+    W.h = W.C.h;
+    const int ctu = b - W.j.first;
+    W.ctu_x = (ctu % W.j.cols) * 64;
+    W.ctu_y = (ctu / W.j.cols) * 64;
+    W.out = calls + static_cast<size_t>(b) * kChainMaxCalls;
+    W.nout = 0;
+    const int t = static_cast<int>(threadIdx.x);
+    if (t < 64) { L.dx[t] = 0; L.dy[t] = 0; L.un[t] = 1; }
+    __syncthreads();
+    chain_split<0>(L, W, 0, 0);
+    if (t == 0) counts[b] = W.nout;
+}
+
 // Sum over each group of L consecutive lanes (L = 1, 2, 4, ..., 64, groups
 // aligned to L); every lane of a group gets its group's total.  DPP
 // butterflies: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
@@ -1609,6 +1795,26 @@ hipError_t launch_list(const PicDev *table, const tcme_call *calls, int n, tcme_
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(me_list_kernel, dim3(n), dim3(256), 0, s, table, calls, res);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain(const PicDev *table, const ChainJobHost *jobs, int njobs, int total_ctus,
+                        tcme_chain_call *calls, int32_t *counts, hipStream_t s)
+{
+    if (total_ctus <= 0) return hipSuccess;
+    ChainArgs a{};
+    a.njobs = njobs;
+    for (int i = 0; i < njobs; ++i) {
+        ChainJob &j = a.job[i];
+        j.cur_pic = jobs[i].cur_pic;
+        j.nrefs = jobs[i].nrefs;
+        for (int r = 0; r < 4; ++r) j.ref_pic[r] = jobs[i].ref_pic[r];
+        j.sr = jobs[i].sr;
+        j.lambda = jobs[i].lambda;
+        j.first = jobs[i].first;
+        j.cols = jobs[i].cols;
+    }
+    hipLaunchKernelGGL(me_chain_kernel, dim3(total_ctus), dim3(256), 0, s, table, a, calls, counts);
     return hipGetLastError();
 }
 

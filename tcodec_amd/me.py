@@ -32,7 +32,7 @@ EXPORTS = (
     "tcme_global_mv", "tcme_sad", "tcme_graph_create", "tcme_graph_launch",
     "tcme_graph_destroy", "tcme_prepare", "tcme_frame_create", "tcme_frame_destroy",
     "tcme_frame_begin", "tcme_frame_end", "tcme_frame_motion_est", "tcme_frame_get_stats",
-    "tcme_global_mv_static",
+    "tcme_global_mv_static", "tcme_search_chain",
 )
 
 CALL_DTYPE = np.dtype([("ref_pic", "<i4"), ("cur_pic", "<i4"), ("px", "<i4"), ("py", "<i4"),
@@ -43,6 +43,11 @@ PRED_DTYPE = np.dtype([("pic", "<i4"), ("mv_x", "<i4"), ("mv_y", "<i4"), ("blk",
 SAD_DTYPE = np.dtype([("pic", "<i4"), ("x", "<i4"), ("y", "<i4"), ("w", "<i4"), ("h", "<i4"),
                       ("other_pic", "<i4"), ("ox", "<i4"), ("oy", "<i4"), ("buf_off", "<i8"),
                       ("buf_stride", "<i4"), ("reserved", "<i4")])
+CHAIN_FRAME_DTYPE = np.dtype([("cur_pic", "<i4"), ("nrefs", "<i4"), ("ref_pic", "<i4", (4,)),
+                              ("sr", "<i4"), ("lambda", "<i4")])
+CHAIN_CALL_DTYPE = np.dtype([(k, "<i4") for k in ("ref", "px", "py", "cx", "cy", "blk", "mv_x",
+                                                   "mv_y", "sad", "reserved")])
+CHAIN_MAX_CALLS = 676
 STATS_DTYPE = np.dtype([("int_evals", "<i4"), ("q6_evals", "<i4"), ("qbil_evals", "<i4"),
                         ("int_oob", "<i4")])
 
@@ -129,6 +134,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tcme_global_mv.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     L.tcme_global_mv_static.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                         C.c_void_p]
+    L.tcme_search_chain.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.c_int64, C.c_void_p]
     L.tcme_sad.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
                            C.c_void_p]
     for f in (L.tcme_predict_luma, L.tcme_predict_chroma):
@@ -285,6 +292,50 @@ class Engine:
         _check(self.L.tcme_search_host(self.h, calls.ctypes.data, n, res.ctypes.data,
                                        st.ctypes.data if stats else None, C.c_void_p(stream)))
         return (res, st) if stats else res
+
+    def chain_ctus(self, pid: int) -> int:
+        w, h = self.pics[pid]
+        return ((w + 63) // 64) * ((h + 63) // 64)
+
+    def chain_device(self, frames: np.ndarray, calls_ptr: int, counts_ptr: int, ctu_cap: int,
+                     stream: int = 0) -> None:
+        """The predictor chain over host frame records (CHAIN_FRAME_DTYPE) into
+        device buffers; asynchronous on ``stream``."""
+        frames = np.ascontiguousarray(frames, CHAIN_FRAME_DTYPE)
+        _check(self.L.tcme_search_chain(self.h, frames.ctypes.data, len(frames),
+                                        C.c_void_p(calls_ptr), C.c_void_p(counts_ptr),
+                                        int(ctu_cap), C.c_void_p(stream)))
+
+    def chain(self, frames):
+        """frames: [(cur_pic, [ref_pic, ...], sr, lambda)] -> per frame an int32
+        [N, 10] array of its searches (ctu, ref, px, py, cx, cy, n, mv_x, mv_y,
+        sad), CTUs in raster order, each CTU's in the order they were made."""
+        import torch
+        rec = np.zeros(len(frames), CHAIN_FRAME_DTYPE)
+        for i, (cur, refs, sr, lam) in enumerate(frames):
+            rec[i]["cur_pic"], rec[i]["nrefs"] = cur, len(refs)
+            rec[i]["ref_pic"][:len(refs)] = refs
+            rec[i]["sr"], rec[i]["lambda"] = sr, lam
+        nct = [self.chain_ctus(int(f[0])) for f in frames]
+        total = sum(nct)
+        d_calls = torch.empty(total * CHAIN_MAX_CALLS * CHAIN_CALL_DTYPE.itemsize,
+                              dtype=torch.uint8, device="cuda")
+        d_counts = torch.full((total,), -1, dtype=torch.int32, device="cuda")
+        self.chain_device(rec, d_calls.data_ptr(), d_counts.data_ptr(), total)
+        torch.cuda.synchronize()
+        counts = d_counts.cpu().numpy()
+        allc = d_calls.cpu().numpy().view(np.int32).reshape(total, CHAIN_MAX_CALLS, 10)
+        out, k0 = [], 0
+        for n in nct:
+            rows = []
+            for k in range(n):
+                c = int(counts[k0 + k])
+                assert 0 <= c <= CHAIN_MAX_CALLS, c
+                r = allc[k0 + k, :c, :9]
+                rows.append(np.concatenate([np.full((c, 1), k, np.int32), r], 1))
+            out.append(np.concatenate(rows) if rows else np.zeros((0, 10), np.int32))
+            k0 += n
+        return out
 
     def tiling(self) -> Tiling:
         t = Tiling()
