@@ -34,7 +34,9 @@ typedef struct tcme_mv { int32_t x, y; } tcme_mv;
 /* Replaces tc_motion_est, reference src/motion.c:384-389 (prototype
  * include/tcodec_common.h:454-459).  Returns the absolute quarter-pel
  * position of the best n x n block and, if best_sad_out is not NULL, its SAD.
- * blk_size must be 8, 16, 32 or 64 (every size the reference encoder uses).
+ * Any blk_size >= 1, as the reference (8, 16, 32 and 64, the sizes its
+ * encoders use, run on kernels specialised for them; other sizes on a
+ * generic kernel).
  * A translation unit that also includes the reference's tcodec_common.h,
  * which declares the same symbol with its own (layout-identical) types,
  * defines TCME_HIDE_DROPIN before including this header. */

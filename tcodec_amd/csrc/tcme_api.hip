@@ -849,6 +849,7 @@ struct DropIn {
     tcme_result *h_res = nullptr;
     const uint8_t *d_blk = nullptr;   // device addresses of the two
     tcme_result *d_res = nullptr;
+    size_t blk_cap = 0;               // bytes of h_blk
     long calls = 0, uploads = 0;
 };
 
@@ -906,6 +907,7 @@ void dropin_init_body()
     }
     d.d_blk = static_cast<const uint8_t *>(db);
     d.d_res = static_cast<tcme_result *>(dr);
+    d.blk_cap = tcme::kMaxBlk * tcme::kMaxBlk + tcme::kTailSlack;
 }
 
 // Device picture holding the bytes of host plane `ref` that the call reading
@@ -966,8 +968,8 @@ TCME_API tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, in
     DropIn &d = g_drop;
     dropin_ensure();
     std::lock_guard<std::mutex> g(d.mu);
-    if (blk_size != 8 && blk_size != 16 && blk_size != 32 && blk_size != 64) {
-        fail(TCME_E_BLK, "block size %d (supported: 8, 16, 32, 64)", blk_size);
+    if (blk_size < 1 || blk_size > tcme::kMaxPicSide) {
+        fail(TCME_E_BLK, "block size %d", blk_size);
         die("tc_motion_est");
     }
     if (ref_w <= 0 || ref_h <= 0 || ref_stride < ref_w) {
@@ -995,6 +997,18 @@ TCME_API tcme_mv tc_motion_est(const uint8_t *ref, int ref_stride, int ref_w, in
     // kernel reads the block from h_blk); the kernel overwrites the sentinel
     // status, so a launch that did not run cannot pass for a result.
     const tcme_call call = {rpic, -1, 0, 0, center_x, center_y, n, search_range};
+    const size_t need = static_cast<size_t>(n) * n + tcme::kTailSlack;
+    if (need > d.blk_cap) {
+        // a block larger than 64x64 (only the generic kernel takes those)
+        if (hipStreamSynchronize(d.stream) != hipSuccess) die("stream synchronize");
+        (void)hipHostFree(d.h_blk);
+        void *db = nullptr;
+        if (hipHostMalloc(reinterpret_cast<void **>(&d.h_blk), need, hipHostMallocCoherent) != hipSuccess ||
+            hipHostGetDevicePointer(&db, d.h_blk, 0) != hipSuccess)
+            die("drop-in block staging");
+        d.d_blk = static_cast<const uint8_t *>(db);
+        d.blk_cap = need;
+    }
     for (int r = 0; r < n; ++r) memcpy(d.h_blk + r * n, orig + static_cast<size_t>(r) * orig_stride, n);
     volatile tcme_result *hr = d.h_res;
     hr->status = TCME_E_HIP;

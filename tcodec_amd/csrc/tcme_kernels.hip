@@ -787,6 +787,124 @@ __global__ __launch_bounds__(64 * WPC) void me_dropin_kernel(const PicDev *__res
     }
 }
 
+// The drop-in's single call for a block size other than 8, 16, 32 or 64 (the
+// reference takes any n, src/motion.c:384-392; its encoders use those four).
+// One workgroup of 256 threads walks the reference's sequential search with
+// wave-uniform decisions; each candidate's SAD is a block-wide sum (each
+// thread a strided share of the n x n samples, then LDS).  The current block
+// (pitch n, mapped host memory) is staged in LDS when n <= 64 and read
+// through otherwise.  Result as me_dropin_kernel: status last, after a
+// system-scope fence.
+constexpr int kGenericLds = 64 * 64;
+
+struct Generic {
+    const PicDev &R;
+    const uint8_t *blk;      // LDS copy (n <= 64) or the host block
+    gptr8 hblk;
+    bool lds;
+    int n, tid;
+    int *part;               // 4 ints of LDS
+    Counts cnt;
+
+    __device__ int cur(int i) const { return lds ? blk[i] : hblk[i]; }
+
+    __device__ int32_t sum(int v)
+    {
+        const int wave = tid >> 6;
+        v = wave_sum(v);
+        __syncthreads();
+        if ((tid & 63) == 0) part[wave] = v;
+        __syncthreads();
+        return part[0] + part[1] + part[2] + part[3];
+    }
+
+    __device__ int32_t sad(int x, int y)
+    {
+        const gptr8 p = as_global8(R.y);
+        int acc = 0;
+        for (int i = tid; i < n * n; i += 256) {
+            const int r = i / n, c = i - r * n;
+            const int d = cur(i) - p[static_cast<size_t>(y + r) * R.pitch + x + c];
+            acc += d < 0 ? -d : d;
+        }
+        return sum(acc);
+    }
+
+    __device__ int32_t qsad(int qx, int qy, int cls)
+    {
+        const int fx = qx >> 2, fy = qy >> 2, dx = qx & 3, dy = qy & 3;
+        const gptr64 an = as_global64(R.an);
+        int acc = 0;
+        for (int i = tid; i < n * n; i += 256) {
+            const int r = i / n, c = i - r * n;
+            const int v = cls == 1
+                ? Searcher<8, 1>::qpix_any(an + static_cast<size_t>(fy + r) * R.apitch + fx + c, R.apitch, dx, dy)
+                : bil_px(R, fx + c, fy + r, dx, dy);
+            const int d = cur(i) - v;
+            acc += d < 0 ? -d : d;
+        }
+        return sum(acc);
+    }
+
+    template <int P>
+    __device__ void stage(int scale, int passes, int &bx, int &by, int32_t &best)
+    {
+        for (int it = 0; it < passes; ++it) {
+            bool moved = false;
+            for (int k = 0; k < Pat<P>::K; ++k) {
+                const int x = bx + Pat<P>::x(k) * scale, y = by + Pat<P>::y(k) * scale;
+                if (!fits(R, n, x, y)) { cnt.oob++; continue; }
+                cnt.ie++;
+                const int32_t s = sad(x, y);
+                if (s < best) { best = s; bx = x; by = y; moved = true; }
+            }
+            if (!moved) break;
+        }
+    }
+};
+
+__global__ __launch_bounds__(256) void me_generic_kernel(const PicDev *__restrict__ table, tcme_call c,
+                                                         const uint8_t *__restrict__ blk,
+                                                         tcme_result *__restrict__ res)
+{
+    __shared__ uint8_t sblk[kGenericLds];
+    __shared__ int part[4];
+    const PicDev R = table[c.ref_pic];
+    const int n = c.blk, tid = static_cast<int>(threadIdx.x);
+    const bool lds = n <= 64;
+    if (lds) {
+        for (int i = tid; i < n * n; i += 256) sblk[i] = as_global8(blk)[i];
+        __syncthreads();
+    }
+    Generic G{R, sblk, as_global8(blk), lds, n, tid, part, Counts{}};
+    // centre clamp with tc_clip's order (src/motion.c:391-392)
+    int bx = c.cx < 0 ? 0 : (c.cx > R.w - n ? R.w - n : c.cx);
+    int by = c.cy < 0 ? 0 : (c.cy > R.h - n ? R.h - n : c.cy);
+    int32_t best = fits(R, n, bx, by) ? G.sad(bx, by) : kInf;
+    int qx = bx * 4, qy = by * 4;
+    if (R.w >= n && R.h >= n) {
+        G.template stage<PAT_LARGE>(search_scale(c.sr, R.w, R.h), 2, bx, by, best);
+        G.template stage<PAT_SMALL>(1, 3, bx, by, best);
+        G.template stage<PAT_DIAMOND>(1, 4, bx, by, best);
+        qx = bx * 4;
+        qy = by * 4;
+        for (int k = 0; k < Pat<PAT_QPEL>::K; ++k) {
+            const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
+            const int cls = qclass(R, n, tx, ty);
+            if (!cls) continue;
+            const int32_t sv = G.qsad(tx, ty, cls);
+            if (sv < best) { best = sv; qx = tx; qy = ty; }
+        }
+    }
+    if (tid == 0) {
+        res->mv_x = qx;
+        res->mv_y = qy;
+        res->sad = best;
+        __threadfence_system();
+        res->status = 0;
+    }
+}
+
 // A short list of calls on resident pictures, one workgroup of 4 wavefronts
 // per call, any block size: the frame-level drop-in's searches of a call
 // that missed its cache, together with its CTU's other nodes at the same
@@ -1785,7 +1903,9 @@ hipError_t launch_dropin(const PicDev *table, const tcme_call &c, const uint8_t 
         hipLaunchKernelGGL((me_dropin_kernel<64, 8>), dim3(1), dim3(512), 0, s, table, c, blk, res);
         break;
     default:
-        return hipErrorInvalidValue;
+        if (c.blk < 1) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(me_generic_kernel, dim3(1), dim3(256), 0, s, table, c, blk, res);
+        break;
     }
     return hipGetLastError();
 }

@@ -239,6 +239,28 @@ def test_dropin_symbol_matches_oracle(orc):
         assert (mx, my, sad) == orc.motion_est(new, cur, 0, 0, 40, 40, 16, 24)
 
 
+def test_dropin_symbol_any_block_size(orc):
+    """tc_motion_est through the drop-in symbol at block sizes the encoders
+    never use (the reference accepts any), including sizes above 64 and a
+    frame smaller than the block: every result equals the oracle's."""
+    from tcodec_amd import me
+    log("drop-in, any block size")
+    rng = np.random.default_rng(33)
+    ref, cur = moving_pair(120, 160, 4, shift=(3, -2))
+    for n in (1, 3, 4, 5, 12, 24, 48, 65, 96, 120):
+        for k in range(10):
+            px = int(rng.integers(0, 160 - n + 1))
+            py = int(rng.integers(0, 120 - n + 1))
+            cx, cy = int(rng.integers(-10, 170)), int(rng.integers(-10, 130))
+            sr = int(rng.choice([1, 8, 16, 24, 32, 64]))
+            (mx, my), sad = me.motion_est(ref, 160, 160, 120, cur, py * 160 + px, 160, cx, cy, n, sr)
+            assert (mx, my, sad) == orc.motion_est(ref, cur, px, py, cx, cy, n, sr), (n, k)
+    # a 24x24 block against a 16x20 reference (no candidate fits)
+    small = np.ascontiguousarray(ref[:20, :16])
+    (mx, my), sad = me.motion_est(small, 16, 16, 20, cur, 0, 160, 3, 2, 24, 16)
+    assert (mx, my, sad) == orc.motion_est(small, cur, 0, 0, 3, 2, 24, 16)
+
+
 TILINGS = [(1, 1, 1, 2, 0, 85, 0), (1, 1, 2, 4, 4, 85, 0), (1, 1, 4, 8, 8, 85, 0),
            (1, 1, 1, 4, 8, 1, 0), (1, 1, 1, 4, 8, 37, 0), (1, 1, 1, 4, 4, 96, 0),
            (1, 1, 1, 4, 4, 85, 1), (1, 1, 1, 4, 4, 7, 1), (1, 1, 1, 4, 8, 85, 1),

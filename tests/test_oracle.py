@@ -76,6 +76,23 @@ def test_motion_est_matches_reference_odd_sizes(orc, ref_lib):
                     assert a == b, (h, w, n, cx, cy, sr, a, b)
 
 
+def test_motion_est_matches_reference_any_block_size(orc, ref_lib):
+    """Block sizes 1..96 (the drop-in accepts every size the reference does)
+    on a frame with texture and motion, centres inside and beyond it."""
+    from helpers import moving_pair
+    rng = np.random.default_rng(15)
+    ref, cur = moving_pair(120, 160, 3, shift=(2, -3))
+    for n in (1, 2, 3, 4, 5, 12, 24, 48, 65, 96, 120):
+        for _ in range(12):
+            px = int(rng.integers(0, 160 - n + 1))
+            py = int(rng.integers(0, 120 - n + 1))
+            cx, cy = int(rng.integers(-10, 170)), int(rng.integers(-10, 130))
+            sr = int(rng.choice([1, 8, 16, 24, 32, 64]))
+            a = orc.motion_est(ref, cur, px, py, cx, cy, n, sr)
+            b = ref_lib.motion_est(ref, cur, px, py, cx, cy, n, sr)
+            assert a == b, (n, px, py, cx, cy, sr, a, b)
+
+
 def test_inter_predict_matches_reference(orc, ref_lib):
     rng = np.random.default_rng(9)
     for trial in range(1500):
