@@ -273,6 +273,34 @@ int  tcme_sad(tcme_ctx *ctx, const tcme_sad_desc *descs, int n, const uint8_t *b
 int  tcme_set_timing(tcme_ctx *ctx, int on);
 int  tcme_last_timing(tcme_ctx *ctx, float *ms, int max);
 
+/* ---- batched time steps ------------------------------------------------ */
+
+/* One picture of a time step: its current picture, references (searched in
+ * this order) and search range. */
+typedef struct tcme_tick_pic {
+    int32_t cur_pic, nrefs, sr;
+    int32_t ref_pic[4];
+} tcme_tick_pic;
+
+/* The batched and served forms' time step from C: every in-frame quadtree
+ * node (64, 32, 16, 8) of each picture against each of its references,
+ * centred at the block position, searched in ONE launch.  The call list is
+ * built on the device from per-shape layouts the context keeps (built once
+ * per picture size, reference count and range); pictures' calls follow one
+ * another, offsets (host, n + 1 entries) receive each picture's first result
+ * index and the total; results (device) holds results_cap entries.  pics is
+ * a host array.  Asynchronous on stream; a context's steps share one call
+ * list, so successive steps go on one stream (or the caller orders them). */
+int  tcme_tick_search(tcme_ctx *ctx, const tcme_tick_pic *pics, int n, tcme_result *results,
+                      int64_t results_cap, int64_t *offsets, void *stream);
+/* A picture shape's call list as tcme_tick_search lays it out (complete CTUs
+ * first, then partial ones at the edges; reference-major; CTU raster order;
+ * largest blocks first inside a CTU, each size in raster order), with
+ * ref_pic = the reference index and cur_pic = -1.  Returns the call count;
+ * calls (host, cap entries) may be NULL, and so may ctx (no device work). */
+int64_t tcme_tick_layout(tcme_ctx *ctx, int w, int h, int nrefs, int sr, tcme_call *calls,
+                         int64_t cap);
+
 /* Host-pointer convenience: copies calls in, searches, copies results out,
  * synchronises the stream. */
 int  tcme_search_host(tcme_ctx *ctx, const tcme_call *calls, int n, tcme_result *results,

@@ -21,8 +21,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <thread>
 #include <vector>
 
@@ -98,6 +100,16 @@ struct tcme_ctx {
     hipEvent_t t0[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t t1[4] = {nullptr, nullptr, nullptr, nullptr};
     int nlaunched = 0;
+    // batched time steps (tcme_tick_search): device layouts per picture shape,
+    // the step's call list and picture table, pinned staging of the table
+    struct Layout { tcme_call *d = nullptr; int64_t n = 0; };
+    std::map<std::tuple<int, int, int, int>, Layout> layouts;
+    tcme_call *d_tick_calls = nullptr;
+    int64_t tick_cap = 0;
+    tcme::TickPicDev *d_tick_pics = nullptr, *h_tick_pics = nullptr;
+    int tick_pics_cap = 0;
+    hipEvent_t tick_copied = nullptr;
+    bool tick_pending = false;
     std::mutex mu;
     std::mutex launch_mu;
 };
@@ -345,6 +357,11 @@ TCME_API void tcme_destroy(tcme_ctx *c)
     if (c->up_done) (void)hipEventDestroy(c->up_done);
     (void)hipFree(c->d_gmv);
     if (c->h_up) (void)hipHostFree(c->h_up);
+    for (auto &kv : c->layouts) (void)hipFree(kv.second.d);
+    (void)hipFree(c->d_tick_calls);
+    (void)hipFree(c->d_tick_pics);
+    if (c->h_tick_pics) (void)hipHostFree(c->h_tick_pics);
+    if (c->tick_copied) (void)hipEventDestroy(c->tick_copied);
     delete c;
 }
 
@@ -769,6 +786,150 @@ TCME_API int tcme_search_chain(tcme_ctx *c, const tcme_chain_frame *frames, int 
     HIP_TRY(tcme::launch_chain(c->d_table, jobs, n, static_cast<int>(total), calls, counts,
                                static_cast<hipStream_t>(stream)));
     return TCME_OK;
+}
+
+namespace {
+
+// The calls of one picture shape as a time step lays them out: every
+// in-frame quadtree node (64, 32, 16, 8) against each reference, centred at
+// the block position; complete CTUs (all 85 nodes in the frame) first, then
+// the partial ones at the edges; reference-major; CTUs in raster order; in a
+// CTU the largest blocks first, each size in raster order.  ref_pic holds the
+// reference index, cur_pic -1.
+std::vector<tcme_call> make_layout(int w, int h, int nrefs, int sr)
+{
+    const int cols = (w + 63) / 64, rows = (h + 63) / 64;
+    std::vector<std::vector<std::pair<int, int>>> nodes(static_cast<size_t>(cols) * rows);
+    std::vector<std::vector<int>> sizes(nodes.size());
+    for (int n = 64; n >= 8; n >>= 1)
+        for (int y = 0; y + n <= h; y += n)
+            for (int x = 0; x + n <= w; x += n) {
+                const size_t k = static_cast<size_t>(y / 64) * cols + x / 64;
+                nodes[k].emplace_back(x, y);
+                sizes[k].push_back(n);
+            }
+    std::vector<tcme_call> out;
+    for (int partial = 0; partial < 2; ++partial)
+        for (int r = 0; r < nrefs; ++r)
+            for (size_t k = 0; k < nodes.size(); ++k) {
+                if ((nodes[k].size() != 85) != (partial == 1)) continue;
+                for (size_t i = 0; i < nodes[k].size(); ++i) {
+                    const int x = nodes[k][i].first, y = nodes[k][i].second, n = sizes[k][i];
+                    out.push_back(tcme_call{r, -1, x, y, x, y, n, sr});
+                }
+            }
+    return out;
+}
+
+}  // namespace
+
+TCME_API int64_t tcme_tick_layout(tcme_ctx *c, int w, int h, int nrefs, int sr, tcme_call *calls,
+                                  int64_t cap)
+{
+    (void)c;  // the layout depends on the shape only (no device work: c may be NULL)
+    if (w <= 0 || h <= 0 || w > tcme::kMaxPicSide || h > tcme::kMaxPicSide || nrefs < 1 ||
+        nrefs > 4 || sr < 0 || sr > 16383)
+        return fail(TCME_E_ARG, "layout %dx%d, %d references, range %d", w, h, nrefs, sr);
+    const std::vector<tcme_call> v = make_layout(w, h, nrefs, sr);
+    if (calls) {
+        if (cap < static_cast<int64_t>(v.size())) return fail(TCME_E_ARG, "room for %lld of %zu calls",
+                                                             (long long)cap, v.size());
+        memcpy(calls, v.data(), v.size() * sizeof(tcme_call));
+    }
+    return static_cast<int64_t>(v.size());
+}
+
+TCME_API int tcme_tick_search(tcme_ctx *c, const tcme_tick_pic *pics, int n, tcme_result *results,
+                              int64_t results_cap, int64_t *offsets, void *stream)
+{
+    if (!c) return fail(TCME_E_ARG, "null context");
+    if (n < 0 || (n > 0 && (!pics || !results || !offsets))) return fail(TCME_E_ARG, "bad tick");
+    if (offsets) offsets[0] = 0;
+    if (n == 0) return TCME_OK;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (set_device(c) != TCME_OK) return TCME_E_HIP;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // validate, find (or build) each picture's layout
+    std::vector<tcme::TickPicDev> tab(n);
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const tcme_tick_pic &p = pics[i];
+        if (p.cur_pic < 0 || p.cur_pic >= c->max_pics || !c->pics[p.cur_pic].used)
+            return fail(TCME_E_PIC, "tick picture %d: no picture %d", i, p.cur_pic);
+        const Pic &cp = c->pics[p.cur_pic];
+        if (p.nrefs < 1 || p.nrefs > 4 || p.sr < 0 || p.sr > 16383)
+            return fail(TCME_E_ARG, "tick picture %d: %d references, range %d", i, p.nrefs, p.sr);
+        tcme::TickPicDev &t = tab[i];
+        for (int r = 0; r < 4; ++r) {
+            const int id = r < p.nrefs ? p.ref_pic[r] : p.ref_pic[0];
+            if (id < 0 || id >= c->max_pics || !c->pics[id].used || !c->pics[id].d_an ||
+                c->pics[id].w != cp.w || c->pics[id].h != cp.h)
+                return fail(TCME_E_PIC, "tick picture %d: reference %d is not a %dx%d reference "
+                            "picture", i, r, cp.w, cp.h);
+            t.ref_pic[r] = id;
+        }
+        const auto key = std::make_tuple(cp.w, cp.h, p.nrefs, p.sr);
+        auto it = c->layouts.find(key);
+        if (it == c->layouts.end()) {
+            const std::vector<tcme_call> v = make_layout(cp.w, cp.h, p.nrefs, p.sr);
+            tcme_ctx::Layout L;
+            L.n = static_cast<int64_t>(v.size());
+            if (L.n > 0) {
+                HIP_TRY(hipStreamSynchronize(s));
+                if (hipMalloc(&L.d, v.size() * sizeof(tcme_call)) != hipSuccess)
+                    return fail(TCME_E_NOMEM, "layout of %lld calls", (long long)L.n);
+                HIP_TRY(hipMemcpy(L.d, v.data(), v.size() * sizeof(tcme_call), hipMemcpyHostToDevice));
+            }
+            it = c->layouts.emplace(key, L).first;
+        }
+        t.tmpl = it->second.d;
+        t.ncalls = static_cast<int32_t>(it->second.n);
+        t.first = total;
+        t.cur_pic = p.cur_pic;
+        total += it->second.n;
+        offsets[i + 1] = total;
+    }
+    if (total > results_cap)
+        return fail(TCME_E_ARG, "%lld calls, room for %lld results", (long long)total,
+                    (long long)results_cap);
+    if (total > INT32_MAX) return fail(TCME_E_ARG, "%lld calls in one step", (long long)total);
+    // the step's call list and picture table (grown on demand; the previous
+    // table copy must be done before its pinned staging is rewritten)
+    if (c->tick_pending) {
+        HIP_TRY(hipEventSynchronize(c->tick_copied));
+        c->tick_pending = false;
+    }
+    if (total > c->tick_cap) {
+        HIP_TRY(hipStreamSynchronize(s));
+        (void)hipFree(c->d_tick_calls);
+        c->d_tick_calls = nullptr;
+        c->tick_cap = 0;
+        if (hipMalloc(&c->d_tick_calls, total * sizeof(tcme_call)) != hipSuccess)
+            return fail(TCME_E_NOMEM, "step call list of %lld calls", (long long)total);
+        c->tick_cap = total;
+    }
+    if (n > c->tick_pics_cap) {
+        HIP_TRY(hipStreamSynchronize(s));
+        (void)hipFree(c->d_tick_pics);
+        if (c->h_tick_pics) (void)hipHostFree(c->h_tick_pics);
+        c->d_tick_pics = nullptr;
+        c->h_tick_pics = nullptr;
+        c->tick_pics_cap = 0;
+        const int cap = std::max(n, 256);
+        if (hipMalloc(&c->d_tick_pics, cap * sizeof(tcme::TickPicDev)) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void **>(&c->h_tick_pics), cap * sizeof(tcme::TickPicDev), 0) !=
+                hipSuccess)
+            return fail(TCME_E_NOMEM, "step picture table");
+        c->tick_pics_cap = cap;
+    }
+    if (!c->tick_copied) HIP_TRY(hipEventCreateWithFlags(&c->tick_copied, hipEventDisableTiming));
+    memcpy(c->h_tick_pics, tab.data(), n * sizeof(tcme::TickPicDev));
+    HIP_TRY(hipMemcpyAsync(c->d_tick_pics, c->h_tick_pics, n * sizeof(tcme::TickPicDev),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(c->tick_copied, s));
+    c->tick_pending = true;
+    HIP_TRY(tcme::launch_expand(c->d_tick_pics, n, c->d_tick_calls, total, s));
+    return dispatch(c, c->d_tick_calls, static_cast<int>(total), c->tiling, results, nullptr, s);
 }
 
 TCME_API int tcme_set_timing(tcme_ctx *c, int on)

@@ -359,6 +359,50 @@ hipError_t launch_global_mv(const PicDev *table, int npics, const int32_t *pairs
     return hipGetLastError();
 }
 
+namespace {
+
+// One call list for a tick's pictures: call i belongs to the picture p whose
+// range [first, first + ncalls) holds it (binary search over the pictures'
+// starts); it is that picture's layout entry i - first with the reference
+// index mapped to the picture id and the current picture filled in.
+__global__ __launch_bounds__(256) void expand_kernel(const TickPicDev *__restrict__ pics, int npics,
+                                                     tcme_call *__restrict__ out, int64_t total)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < total;
+         i += static_cast<int64_t>(gridDim.x) * 256) {
+        int lo = 0, hi = npics - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pics[mid].first <= i) lo = mid; else hi = mid - 1;
+        }
+        const TickPicDev &p = pics[lo];
+        // the template pointer was read from memory: loaded through a
+        // global view (two 16-byte words), so the load is not a generic one
+        typedef unsigned int w4 __attribute__((ext_vector_type(4)));
+        const __attribute__((address_space(1))) w4 *q =
+            (const __attribute__((address_space(1))) w4 *)(p.tmpl) + 2 * (i - p.first);
+        const w4 a = q[0], b = q[1];
+        tcme_call c{static_cast<int32_t>(a.x), static_cast<int32_t>(a.y), static_cast<int32_t>(a.z),
+                    static_cast<int32_t>(a.w), static_cast<int32_t>(b.x), static_cast<int32_t>(b.y),
+                    static_cast<int32_t>(b.z), static_cast<int32_t>(b.w)};
+        c.ref_pic = p.ref_pic[c.ref_pic];
+        c.cur_pic = p.cur_pic;
+        out[i] = c;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_expand(const TickPicDev *pics, int npics, tcme_call *out, int64_t total,
+                         hipStream_t s)
+{
+    if (total <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((total + 255) / 256, 16 * device_cus());
+    hipLaunchKernelGGL(expand_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, pics,
+                       npics, out, total);
+    return hipGetLastError();
+}
+
 hipError_t launch_sad(const PicDev *table, int npics, const tcme_sad_desc *descs, int n,
                       const uint8_t *buf, int64_t buf_bytes, int32_t *out, hipStream_t s)
 {

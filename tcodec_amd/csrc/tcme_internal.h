@@ -68,6 +68,17 @@ hipError_t launch_anchors(const PicDev &p, hipStream_t s);
 // followed by at least kTailSlack readable bytes.
 hipError_t launch_dropin(const PicDev *table, const tcme_call &c, const uint8_t *blk,
                          tcme_result *res, hipStream_t s);
+// A picture of a tick (tcme_tick_search): its layout (device template whose
+// ref_pic fields are reference indices), picture ids and place in the list.
+struct TickPicDev {
+    const tcme_call *tmpl;
+    int64_t first;
+    int32_t ncalls, cur_pic;
+    int32_t ref_pic[4];
+};
+hipError_t launch_expand(const TickPicDev *pics, int npics, tcme_call *out, int64_t total,
+                         hipStream_t s);
+
 // The ME-only predictor chain: up to kChainFrames frames per launch, one
 // workgroup per CTU (frames' CTUs numbered from `first`, raster order, `cols`
 // CTUs per row), kChainMaxCalls records per CTU in `calls`.  Jobs are

@@ -32,7 +32,7 @@ EXPORTS = (
     "tcme_global_mv", "tcme_sad", "tcme_graph_create", "tcme_graph_launch",
     "tcme_graph_destroy", "tcme_prepare", "tcme_frame_create", "tcme_frame_destroy",
     "tcme_frame_begin", "tcme_frame_end", "tcme_frame_motion_est", "tcme_frame_get_stats",
-    "tcme_global_mv_static", "tcme_search_chain",
+    "tcme_global_mv_static", "tcme_search_chain", "tcme_tick_search", "tcme_tick_layout",
 )
 
 CALL_DTYPE = np.dtype([("ref_pic", "<i4"), ("cur_pic", "<i4"), ("px", "<i4"), ("py", "<i4"),
@@ -48,6 +48,8 @@ CHAIN_FRAME_DTYPE = np.dtype([("cur_pic", "<i4"), ("nrefs", "<i4"), ("ref_pic", 
 CHAIN_CALL_DTYPE = np.dtype([(k, "<i4") for k in ("ref", "px", "py", "cx", "cy", "blk", "mv_x",
                                                    "mv_y", "sad", "reserved")])
 CHAIN_MAX_CALLS = 676
+TICK_PIC_DTYPE = np.dtype([("cur_pic", "<i4"), ("nrefs", "<i4"), ("sr", "<i4"),
+                           ("ref_pic", "<i4", (4,))])
 STATS_DTYPE = np.dtype([("int_evals", "<i4"), ("q6_evals", "<i4"), ("qbil_evals", "<i4"),
                         ("int_oob", "<i4")])
 
@@ -134,6 +136,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     L.tcme_global_mv.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     L.tcme_global_mv_static.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                         C.c_void_p]
+    L.tcme_tick_search.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64,
+                                   C.c_void_p, C.c_void_p]
+    L.tcme_tick_layout.restype = C.c_int64
+    L.tcme_tick_layout.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                   C.c_int64]
     L.tcme_search_chain.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                     C.c_int64, C.c_void_p]
     L.tcme_sad.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
@@ -293,6 +300,15 @@ class Engine:
                                        st.ctypes.data if stats else None, C.c_void_p(stream)))
         return (res, st) if stats else res
 
+    def tick_search(self, recs: np.ndarray, res_ptr: int, res_cap: int, stream: int = 0):
+        """One batched time step (tcme_tick_search) over host tcme_tick_pic
+        records into a device result buffer; returns the offsets (n + 1)."""
+        recs = np.ascontiguousarray(recs, TICK_PIC_DTYPE)
+        offs = np.zeros(len(recs) + 1, np.int64)
+        _check(self.L.tcme_tick_search(self.h, recs.ctypes.data, len(recs), C.c_void_p(res_ptr),
+                                       int(res_cap), offs.ctypes.data, C.c_void_p(stream)))
+        return offs
+
     def chain_ctus(self, pid: int) -> int:
         w, h = self.pics[pid]
         return ((w + 63) // 64) * ((h + 63) // 64)
@@ -395,6 +411,16 @@ class Graph:
         if self.h:
             self.eng.L.tcme_graph_destroy(self.h)
             self.h = None
+
+
+def tick_layout(w: int, h: int, nrefs: int, sr: int) -> np.ndarray:
+    """A picture shape's call list as tcme_tick_search lays it out (CALL_DTYPE,
+    ref_pic = reference index, cur_pic = -1); host only, no GPU needed."""
+    L = load()
+    n = _check(L.tcme_tick_layout(None, w, h, nrefs, sr, None, 0))
+    out = np.zeros(n, CALL_DTYPE)
+    _check(L.tcme_tick_layout(None, w, h, nrefs, sr, out.ctypes.data, n))
+    return out
 
 
 def make_calls(ref_pic, cur_pic, px, py, cx, cy, blk, sr) -> np.ndarray:

@@ -207,6 +207,7 @@ class Stream:
     ring: list = field(default_factory=list)   # device picture ids
     slot_of: dict = field(default_factory=dict)  # poc -> ring index
     ready_at: float = 0.0
+    busy: bool = False            # a picture of this stream is in flight
     latencies: list = field(default_factory=list)
     results: list = field(default_factory=list)   # keep_results=True: every picture's results
     digests: list = field(default_factory=list)   # keep_results="digest": one per picture
@@ -285,6 +286,16 @@ def expand_calls(tpl: Templates, items) -> tuple[torch.Tensor, list[int]]:
     return (torch.cat(parts) if len(parts) > 1 else parts[0]), sizes
 
 
+def tick_records(items) -> np.ndarray:
+    """tcme_tick_pic records of a tick's pictures (items as Stream.prepare
+    returns them)."""
+    rec = np.zeros(len(items), me.TICK_PIC_DTYPE)
+    for i, ((w, h, nrefs, sr), lut) in enumerate(items):
+        rec[i]["cur_pic"], rec[i]["nrefs"], rec[i]["sr"] = lut[-1], nrefs, sr
+        rec[i]["ref_pic"][:nrefs] = lut[:-1]
+    return rec
+
+
 def digests(res: torch.Tensor, sizes: list[int]) -> torch.Tensor:
     """One int64 digest per picture of a tick's results (int32 [N, 4], the
     pictures' calls concatenated): a sum of every field weighted by the
@@ -306,7 +317,8 @@ class Server:
     """Continuous-batching request loop over one tcme context."""
 
     def __init__(self, eng: me.Engine, hbm_budget: int, max_tick_calls: int = 6_000_000,
-                 keep_results: bool = False, per_stream_launch: bool = False, trace=None):
+                 keep_results: bool = False, per_stream_launch: bool = False, trace=None,
+                 pipeline: bool = False, gpu_timing: bool = False):
         self.eng = eng
         self.trace = trace        # trace.TraceWriter: record planes and searches
         self.per_stream = per_stream_launch
@@ -324,6 +336,10 @@ class Server:
         self.calls = 0
         self.ticks = 0
         self.tick_ms: list[float] = []
+        self.pipeline = pipeline
+        self.gpu_timing = gpu_timing  # events around each launch: gpu_ms
+        self.gpu_ms = 0.0             # device time of the launches' work
+        self.inflight: list = []
         self.max_active = 0           # most streams searched by one launch
         self.admitted_while_busy = 0  # admissions joining streams already running
 
@@ -339,6 +355,23 @@ class Server:
 
     def submit(self, s: Stream) -> None:
         self.queue.append(s)
+
+    _layout_n: dict = {}
+
+    @classmethod
+    def layout_size(cls, key) -> int:
+        n = cls._layout_n.get(key)
+        if n is None:
+            n = cls._layout_n[key] = len(frame_template(key[0], key[1], key[2], key[3]))
+        return n
+
+    @staticmethod
+    def host_calls(key, lut) -> np.ndarray:
+        """A picture's call list on the host (as the library lays it out)."""
+        calls = frame_template(*key).copy()
+        calls["ref_pic"] = np.asarray(lut[:-1], np.int32)[calls["ref_pic"]]
+        calls["cur_pic"] = lut[-1]
+        return calls
 
     _nodes: dict = {}
 
@@ -374,38 +407,63 @@ class Server:
             self.active.append(s)
             self.max_active = max(self.max_active, len(self.active))
 
-    def tick(self, now: float) -> None:
-        """One launch over the next picture of every active stream."""
+    def _launch(self, now: float, streams: list) -> dict:
+        """Enqueues one launch over the next picture of each of `streams`
+        (uploads, anchors and the search, on the server's stream) and returns
+        what completing it needs."""
         items, owners = [], []
-        for s in self.active:
+        for s in streams:
             pos = s.pos
             c = s.prepare(self.eng, self.sptr, self.trace)
+            s.busy = True
             if c is not None:
                 items.append(c)
                 owners.append((s, pos))
-        res = None
+        t = {"now": now, "streams": streams, "items": items, "owners": owners, "res": None,
+             "sizes": [], "dig": None, "ev": None}
+        if self.gpu_timing:
+            t["ev"] = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            t["ev"][0].record(self.stream)
         if items:
-            calls, sizes = expand_calls(self.tpl, items)
-            res = torch.empty((calls.shape[0], 4), dtype=torch.int32, device="cuda")
+            # the step's call list is built on the device by the library
+            # (tcme_tick_search) from per-shape layouts
+            recs = tick_records(items)
+            sizes = [self.layout_size(key) for key, _ in items]
+            res = torch.empty((sum(sizes), 4), dtype=torch.int32, device="cuda")
             if self.per_stream:
                 # the plain form: one launch per picture, back to back
                 o = 0
-                for n in sizes:
-                    self.eng.search_device(calls[o].data_ptr(), n, res[o].data_ptr(), None,
-                                           self.sptr)
+                for k, n in enumerate(sizes):
+                    self.eng.tick_search(recs[k:k + 1], res[o].data_ptr(), n, self.sptr)
                     o += n
             else:
-                self.eng.search_device(calls.data_ptr(), calls.shape[0], res.data_ptr(), None,
-                                       self.sptr)
-            dig = digests(res, sizes) if self.keep == "digest" else None
-        self.stream.synchronize()
+                self.eng.tick_search(recs, res.data_ptr(), res.shape[0], self.sptr)
+            t["res"], t["sizes"] = res, sizes
+            t["dig"] = digests(res, sizes) if self.keep == "digest" else None
+        if t["ev"] is not None:
+            t["ev"][1].record(self.stream)
+        done = torch.cuda.Event()
+        done.record(self.stream)
+        t["done"] = done
+        self.inflight.append(t)
+        return t
+
+    def _complete(self, t: dict) -> None:
+        """Waits for a launch and books its results, latencies and finished
+        streams."""
+        t["done"].synchronize()
         end = time.perf_counter()
+        self.inflight.remove(t)
+        items, owners, res, sizes = t["items"], t["owners"], t["res"], t["sizes"]
+        if t["ev"] is not None:
+            self.gpu_ms += t["ev"][0].elapsed_time(t["ev"][1])
         if items and self.trace is not None:
             from .trace import rows_from
-            self.trace.calls(rows_from(calls.cpu().numpy(), res.cpu().numpy()))
+            calls = np.concatenate([self.host_calls(key, lut) for key, lut in items])
+            self.trace.calls(rows_from(calls.view(np.int32).reshape(-1, 8), res.cpu().numpy()))
         if items and self.keep:
             host = res.cpu().numpy() if self.keep is True else None
-            dh = dig.cpu().numpy() if dig is not None else None
+            dh = t["dig"].cpu().numpy() if t["dig"] is not None else None
             o = 0
             for k, ((s, pos), n) in enumerate(zip(owners, sizes)):
                 if host is not None:
@@ -416,10 +474,10 @@ class Server:
                         s.kept[pos] = res[o:o + n].cpu().numpy()
                 o += n
         self.ticks += 1
-        self.tick_ms.append((end - now) * 1e3)
-        searched = {id(s): n for (s, _), n in zip(owners, sizes)} if items else {}
-        still = []
-        for s in self.active:
+        self.tick_ms.append((end - t["now"]) * 1e3)
+        searched = {id(s): n for (s, _), n in zip(owners, sizes)}
+        for s in t["streams"]:
+            s.busy = False
             if id(s) in searched:
                 s.latencies.append((end - s.ready_at) * 1e3)
                 self.frames += 1
@@ -429,28 +487,42 @@ class Server:
                 s.release(self.eng)
                 self.used -= stream_bytes(s.w, s.h)
                 self.finished.append(s)
-            else:
-                still.append(s)
-        self.active = still
+                self.active.remove(s)
+
+    def tick(self, now: float) -> None:
+        """One launch over the next picture of every active stream, waited for."""
+        self._complete(self._launch(now, list(self.active)))
 
     def run(self, t0: float | None = None) -> float:
         """Serves until every submitted request is finished; arrival times are
-        seconds after t0.  Returns the wall time used."""
+        seconds after t0.  Returns the wall time used.
+
+        Without pipelining each launch is waited for before the next is
+        prepared.  With it (pipeline=True) two launches are in flight: while
+        one runs on the GPU the host prepares and enqueues the next from the
+        streams that are not in it (a stream's pictures stay one at a time:
+        its next picture waits for its previous one), so host work overlaps
+        device work and the streams fall into two alternating groups."""
         self.queue.sort(key=lambda s: s.arrival)
         t0 = time.perf_counter() if t0 is None else t0
         for s in self.queue:
             s.arrival += t0
-        while self.queue or self.active:
+        depth = 2 if self.pipeline else 1
+        while self.queue or self.active or self.inflight:
             now = time.perf_counter()
             self._admit(now)
-            if not self.active:
-                if not self.queue:
-                    break
-                wait = self.queue[0].arrival - now
-                if wait > 0:
-                    time.sleep(min(wait, 0.05))
+            ready = [s for s in self.active if not s.busy]
+            if ready and len(self.inflight) < depth:
+                self._launch(now, ready)
                 continue
-            self.tick(now)
+            if self.inflight:
+                self._complete(self.inflight[0])
+                continue
+            if not self.queue:
+                break
+            wait = self.queue[0].arrival - now
+            if wait > 0:
+                time.sleep(min(wait, 0.05))
         return time.perf_counter() - t0
 
     def latencies(self) -> np.ndarray:

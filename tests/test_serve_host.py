@@ -24,3 +24,21 @@ def test_request_larger_than_budget_is_refused_not_blocking():
 def test_stream_bytes_counts_five_resident_pictures():
     assert serve.stream_bytes(1920, 1080) == 5 * serve.pic_bytes(1920, 1080)
     assert serve.pic_bytes(320, 240) == 320 * 240 * 9 + 512
+
+
+def test_library_step_layout_equals_python_template():
+    """tcme_tick_layout (the call list the library builds on the device for a
+    time step) is exactly the Python frame template the oracle checks use:
+    complete CTUs first, reference-major, largest blocks first."""
+    import numpy as np
+
+    from tcodec_amd import me
+    for (w, h, refs, sr) in [(320, 240, 4, 24), (1280, 720, 2, 32), (200, 136, 1, 16),
+                             (64, 64, 3, 24), (1920, 1080, 4, 32), (40, 24, 1, 24)]:
+        a = me.tick_layout(w, h, refs, sr)
+        b = serve.frame_template(w, h, refs, sr)
+        assert len(a) == len(b), (w, h)
+        for f in me.CALL_DTYPE.names:
+            if f != "cur_pic":
+                assert np.array_equal(a[f], b[f]), (w, h, f)
+        assert (a["cur_pic"] == -1).all()
