@@ -172,17 +172,17 @@ def test_config4_64_stream_mix_one_launch_per_time_step(engine, orc):
         for sid, w, h, clip in clips:
             srv.submit(serve.Stream(sid, w, h, nf, "P", clip, keep_at={nf - 1}))
         launches = []
-        orig = engine.search_device
+        orig = engine.tick_search
 
-        def counted(*a, **k):
-            launches.append(a[1])
-            return orig(*a, **k)
+        def counted(recs, res_ptr, res_cap, stream=0):
+            launches.append(res_cap)          # one tcme_tick_search: one search launch
+            return orig(recs, res_ptr, res_cap, stream)
 
-        engine.search_device = counted
+        engine.tick_search = counted
         try:
             srv.run()
         finally:
-            del engine.search_device
+            del engine.tick_search
         assert len(srv.finished) == 64 and srv.ticks == nf and srv.max_active == 64
         return srv, launches
 
