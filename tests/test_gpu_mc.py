@@ -43,6 +43,45 @@ def test_gpu_luma_mc_matches_oracle(engine, orc, shape):
     engine.free(rp)
 
 
+def test_gpu_luma_mc_8x8_quads(engine, orc):
+    """Runs of 8x8 blocks (as a frame's call list holds them), predicted four
+    to a wavefront: every phase, integer MVs, quads whose blocks are all in
+    the interior, quads with one block on or past the interior's edge (the
+    whole quad then takes the per-block path), lists whose length is not a
+    multiple of four, and a rejected descriptor inside a quad."""
+    from tcodec_amd import me, synth
+    print("luma mc 8x8 quads", flush=True)
+    h, w = 240, 320
+    ref = synth.Scene(w, h, seed=12, rects=10).frame(1)[0]
+    rp = engine.pic(ref)
+    rng = np.random.default_rng(8)
+    for count in (4001, 4000, 6):
+        sizes = np.full(count, 8)
+        fx = rng.integers(2, w - 8 - 2, count)
+        fy = rng.integers(2, h - 8 - 2, count)
+        mvs = np.stack([fx * 4 + rng.integers(0, 4, count), fy * 4 + rng.integers(0, 4, count)], 1)
+        mvs[::7] &= ~3                                        # integer MVs
+        if count > 100:
+            # one block per some quads at the interior's edge or beyond it
+            for k in range(1, count, 37):
+                mvs[k] = [(w - 8 - 3 + int(rng.integers(0, 3))) * 4 + int(rng.integers(0, 4)),
+                          int(rng.integers(-40, 8 * 4))]
+        preds = _preds(rp, mvs, sizes)
+        if count > 100:
+            preds["pic"][50] = 999                            # rejected inside a quad
+        out, st = engine.predict(preds)
+        for k in range(len(preds)):
+            if count > 100 and k == 50:
+                assert st[k] == -4
+                continue
+            assert st[k] == 0
+            o = int(preds["out_off"][k])
+            got = out[o:o + 64].reshape(8, 8)
+            want = orc.inter_predict(ref, int(mvs[k, 0]), int(mvs[k, 1]), 8)
+            assert np.array_equal(got, want), (count, k, mvs[k])
+    engine.free(rp)
+
+
 def test_gpu_luma_mc_interior_blocks(engine, orc):
     """Blocks whose every sample has its 6x6 apron inside the frame (the
     kernel's batched path), each of the 16 phases, block sizes that do and do
