@@ -54,9 +54,9 @@ CASES = {
         "scene", 1280, 720, 30, {"rects": 16, "pan": (0.0, 0.0)},
         ["--v2", "--profile", "1", "-p", "2", "-q", "27"], ("slot1",)),
     # the 1080p configuration's generator (48 rectangles, pan (0.75, 0.25)),
-    # 9 frames: two GOP4 ladders of hierarchical B pictures, QP 32
-    "config3_scene_1920x1080_9f_bgop4_q32": (
-        "scene", 1920, 1080, 9, {"rects": 48, "pan": (0.75, 0.25)},
+    # 17 frames: four GOP4 ladders of hierarchical B pictures, QP 32
+    "config3_scene_1920x1080_17f_bgop4_q32": (
+        "scene", 1920, 1080, 17, {"rects": 48, "pan": (0.75, 0.25)},
         ["--v2", "--profile", "1", "-p", "2", "-q", "32", "-b"], ("bframe",)),
     # v1 with 4 threads and no entropy coding: WPP rows on the encoder's
     # thread pool call tc_motion_est concurrently (8x8 stack blocks, stride 8)
