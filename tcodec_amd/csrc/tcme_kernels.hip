@@ -215,8 +215,9 @@ struct LumaWin {
 // Full-pel candidate search + quarter-pel refinement of one call by WPC
 // wavefronts.  qw ((N+2)^2 anchors) and red ([2][WPC][8] ints) are LDS areas
 // owned by this call's wavefronts.  With WIN, integer candidates that lie
-// inside the LDS luma window are read from it instead of the plane.
-template <int N, int WPC, bool WIN = false>
+// inside the LDS luma window are read from it instead of the plane.  Without
+// QWIN the quarter-pel pass reads the anchor plane directly and qw is unused.
+template <int N, int WPC, bool WIN = false, bool QWIN = true>
 struct Searcher {
     using Gm = Geo<N, WPC>;
     const PicDev &R;
@@ -243,8 +244,14 @@ struct Searcher {
           cblk(static_cast<size_t>(py) * cur.pitch + px), cpitch(cur.pitch),
           rplane(as_global32(ref.y)), aplane(as_global64(ref.an))
     {
-        lane = threadIdx.x & 63;
-        tid = static_cast<int>(threadIdx.x) & (64 * WPC - 1);
+        // An empty asm makes the lane index opaque, so the lane-derived
+        // offsets of each search stay inside the loop that runs it instead of
+        // being hoisted and held live across every search of a kernel (which
+        // cost the predictor chain ~100 registers).
+        int t = static_cast<int>(threadIdx.x);
+        asm volatile("" : "+v"(t));
+        lane = t & 63;
+        tid = t & (64 * WPC - 1);
         wave = WPC == 1 ? 0 : __builtin_amdgcn_readfirstlane(tid >> 6);
         grp = Gm::G == 1 ? 0 : lane / Gm::UNITS;
 #pragma unroll
@@ -617,7 +624,7 @@ struct Searcher {
         constexpr int K = Pat<PAT_QPEL>::K;
         qx = bx * 4;
         qy = by * 4;
-        const bool fast = bx - 1 >= 2 && by - 1 >= 2 && bx + N + 2 < R.w && by + N + 2 < R.h;
+        const bool fast = QWIN && bx - 1 >= 2 && by - 1 >= 2 && bx + N + 2 < R.w && by + N + 2 < R.h;
         if (!fast) {
             qpel_direct(bx, by, qx, qy, best);
             return;
@@ -639,12 +646,12 @@ struct Searcher {
     }
 };
 
-template <int N, int WPC, bool WIN = false>
+template <int N, int WPC, bool WIN = false, bool QWIN = true>
 __device__ void search(const PicDev &R, const PicDev &C, const tcme_call &c, tcme_result *res,
                        tcme_stats *st, anc_t *qw, int *red,
                        LumaWin lw = LumaWin{nullptr, 0, 0, 0, 0})
 {
-    Searcher<N, WPC, WIN> S(R, C, c.px, c.py, qw, red, lw);
+    Searcher<N, WPC, WIN, QWIN> S(R, C, c.px, c.py, qw, red, lw);
     // Centre clamp with tc_clip's order: low bound first (motion.c:391-392).
     int bx = c.cx < 0 ? 0 : (c.cx > R.w - N ? R.w - N : c.cx);
     int by = c.cy < 0 ? 0 : (c.cy > R.h - N ? R.h - N : c.cy);
@@ -957,7 +964,7 @@ __global__ __launch_bounds__(256) void me_list_kernel(const PicDev *__restrict__
 }
 
 // ---- ME-only predictor chain ------------------------------------------------
-// One workgroup of 4 wavefronts per CTU walks the CTU's quadtree in the
+// One wavefront per CTU walks the CTU's quadtree in the
 // reference encoder's decision order (src/encoder.c:1292-1397: a node as a
 // leaf, its four children as leaves, and, when the children's cost is not
 // above the parent's, each child again as a subtree; nodes not wholly in the
@@ -966,9 +973,10 @@ __global__ __launch_bounds__(256) void me_list_kernel(const PicDev *__restrict__
 // "unavailable" per CTU (:1425-1428); it searches each reference in turn and
 // keeps the cheapest J = SAD + lambda * (se_bits(mvd.x) + se_bits(mvd.y))
 // (first on ties), and its cells take the chosen displacement.  The control
-// flow is uniform: every thread holds the same walk state; the grid lives in
-// LDS.  8x8 and 16x16 searches run on the first wavefront, larger ones on all
-// four.
+// flow is uniform: every lane holds the same walk state; the grid lives in
+// LDS.  A CTU's searches run one after another, so wider workgroups would
+// mostly wait; one wavefront keeps the LDS per CTU small (the 64x64
+// quarter-pel pass reads the anchor plane directly) and many CTUs resident.
 struct ChainJob {
     int32_t cur_pic, nrefs, ref_pic[4], sr, lambda, first, cols;
 };
@@ -977,18 +985,25 @@ struct ChainArgs {
     ChainJob job[kChainFrames];
 };
 
+// One level of the quadtree walk: the node's first cell, its stage, the
+// next child, and the costs of the node and of its children as leaves.
+struct ChainLevel {
+    int32_t cx, cy, stage, q;
+    int64_t parent, child;
+};
+
 struct ChainLds {
     union {
         Shared<8, 1> s8;
         Shared<16, 1> s16;
-        Shared<32, 4> s32;
-        Shared<64, 4> s64;
+        Shared<32, 1> s32;
     } sh;
     tcme_result r;
     int16_t dx[64], dy[64];
     uint8_t un[64];                          // 1: unavailable (intra)
     int16_t sdx[2][3][64], sdy[2][3][64];    // saved regions per depth: pre, parent
     uint8_t sun[2][3][64];
+    ChainLevel lv[4];                        // walk stack, one level per depth
 };
 
 struct ChainWalk {
@@ -1049,22 +1064,18 @@ __device__ __forceinline__ int64_t chain_leaf(ChainLds &L, ChainWalk &W, int cx,
     int mx, my;
     chain_mvp(L, cx, cy, mx, my);
     const int ccx = (mx + 4 * px) >> 2, ccy = (my + 4 * py) >> 2;
-    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     int64_t best = INT64_MAX;
     int bdx = 0, bdy = 0;
     for (int r = 0; r < W.j.nrefs; ++r) {
-        const PicDev R = W.table[W.j.ref_pic[r]];
-        const tcme_call c{W.j.ref_pic[r], W.j.cur_pic, px, py, ccx, ccy, N, W.j.sr};
-        if constexpr (N <= 16) {
-            if (wave == 0) {
-                if constexpr (N == 8) search<8, 1>(R, W.C, c, &L.r, nullptr, L.sh.s8.win, &L.sh.s8.red[0][0][0]);
-                else search<16, 1>(R, W.C, c, &L.r, nullptr, L.sh.s16.win, &L.sh.s16.red[0][0][0]);
-            }
-        } else if constexpr (N == 32) {
-            search<32, 4>(R, W.C, c, &L.r, nullptr, L.sh.s32.win, &L.sh.s32.red[0][0][0]);
-        } else {
-            search<64, 4>(R, W.C, c, &L.r, nullptr, L.sh.s64.win, &L.sh.s64.red[0][0][0]);
-        }
+        // a select, not an index: indexing the walk state by r would keep it
+        // in private memory
+        const int rp = r == 0 ? W.j.ref_pic[0] : r == 1 ? W.j.ref_pic[1] : r == 2 ? W.j.ref_pic[2] : W.j.ref_pic[3];
+        const PicDev R = W.table[rp];
+        const tcme_call c{rp, W.j.cur_pic, px, py, ccx, ccy, N, W.j.sr};
+        if constexpr (N == 8) search<8, 1>(R, W.C, c, &L.r, nullptr, L.sh.s8.win, &L.sh.s8.red[0][0][0]);
+        else if constexpr (N == 16) search<16, 1>(R, W.C, c, &L.r, nullptr, L.sh.s16.win, &L.sh.s16.red[0][0][0]);
+        else if constexpr (N == 32) search<32, 1>(R, W.C, c, &L.r, nullptr, L.sh.s32.win, &L.sh.s32.red[0][0][0]);
+        else search<64, 1, false, false>(R, W.C, c, &L.r, nullptr, nullptr, L.sh.s8.red[0][0]);
         __syncthreads();
         const int mvx = L.r.mv_x, mvy = L.r.mv_y, sad = L.r.sad;
         __syncthreads();  // r and the search's LDS are reused by the next search
@@ -1087,36 +1098,106 @@ __device__ __forceinline__ int64_t chain_leaf(ChainLds &L, ChainWalk &W, int cx,
     return best;
 }
 
-template <int D>
-__device__ __forceinline__ void chain_split(ChainLds &L, ChainWalk &W, int cx, int cy)
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ int64_t uni64(int64_t v)
 {
-    constexpr int N = 64 >> D, half = N / 16;
-    const int px = W.ctu_x + 8 * cx, py = W.ctu_y + 8 * cy;
-    const bool in_frame = px + N <= W.w && py + N <= W.h;
-    if constexpr (D == 3) {
-        if (in_frame) chain_leaf<3>(L, W, cx, cy);
-    } else {
-        if (!in_frame) {
-            for (int q = 0; q < 4; ++q) chain_split<D + 1>(L, W, cx + (q & 1) * half, cy + (q >> 1) * half);
-            return;
+    const uint64_t u = static_cast<uint64_t>(v);
+    const uint32_t hi = static_cast<uint32_t>(uni(static_cast<int>(u >> 32)));
+    const uint32_t lo = static_cast<uint32_t>(uni(static_cast<int>(u)));
+    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Walks the CTU's quadtree with an explicit stack of levels in LDS (every
+// lane keeps the same copy), so that each block size's search appears once
+// in the code.  A level stands at one of these stages:
+enum : int32_t {
+    kEnter,        // node just reached
+    kPop,          // 8x8 leaf searched: back to the parent
+    kOutside,      // node not wholly in the frame: children as subtrees
+    kAfterParent,  // node searched as a leaf: now its children as leaves
+    kChildLeaves,  // children as leaves, q of them requested
+    kSubtrees,     // children again, as subtrees
+};
+
+__device__ __forceinline__ void chain_walk(ChainLds &L, ChainWalk &W)
+{
+    int d = 0;
+    L.lv[0] = ChainLevel{0, 0, kEnter, 0, 0, 0};
+    int64_t cost = 0;  // of the last leaf searched
+    for (;;) {
+        int leaf = -1, lcx = 0, lcy = 0;
+        while (leaf < 0 && d >= 0) {
+            ChainLevel &v = L.lv[d];
+            const int cx = uni(v.cx), cy = uni(v.cy), stage = uni(v.stage), q = uni(v.q);
+            const int n = 64 >> d, half = n / 16, cs = n / 8;
+            const int px = W.ctu_x + 8 * cx, py = W.ctu_y + 8 * cy;
+            const bool in_frame = px + n <= W.w && py + n <= W.h;
+            switch (stage) {
+            case kEnter:
+                if (d == 3) {
+                    if (in_frame) { leaf = 3; lcx = cx; lcy = cy; v.stage = kPop; }
+                    else --d;
+                } else if (!in_frame) {
+                    v.stage = kOutside;
+                    v.q = 0;
+                } else {
+                    chain_copy(L, cx, cy, cs, 0, d, true);
+                    leaf = d; lcx = cx; lcy = cy;
+                    v.stage = kAfterParent;
+                }
+                break;
+            case kPop:
+                --d;
+                break;
+            case kOutside:
+            case kSubtrees:
+                if (q < 4) {
+                    v.q = q + 1;
+                    L.lv[d + 1] = ChainLevel{cx + (q & 1) * half, cy + (q >> 1) * half, kEnter, 0, 0, 0};
+                    ++d;
+                } else {
+                    --d;
+                }
+                break;
+            case kAfterParent:
+                v.parent = cost;
+                chain_copy(L, cx, cy, cs, 1, d, true);
+                chain_copy(L, cx, cy, cs, 0, d, false);
+                v.child = 0;
+                v.q = 1;
+                v.stage = kChildLeaves;
+                leaf = d + 1; lcx = cx; lcy = cy;
+                break;
+            default: {  // kChildLeaves
+                const int64_t child = uni64(v.child) + cost;
+                v.child = child;
+                if (q < 4) {
+                    v.q = q + 1;
+                    leaf = d + 1; lcx = cx + (q & 1) * half; lcy = cy + (q >> 1) * half;
+                } else if (child <= uni64(v.parent)) {
+                    chain_copy(L, cx, cy, cs, 0, d, false);
+                    v.stage = kSubtrees;
+                    v.q = 0;
+                } else {
+                    chain_copy(L, cx, cy, cs, 1, d, false);
+                    --d;
+                }
+                break;
+            }
+            }
         }
-        constexpr int cs = N / 8;
-        chain_copy(L, cx, cy, cs, 0, D, true);
-        const int64_t parent = chain_leaf<D>(L, W, cx, cy);
-        chain_copy(L, cx, cy, cs, 1, D, true);
-        chain_copy(L, cx, cy, cs, 0, D, false);
-        int64_t child = 0;
-        for (int q = 0; q < 4; ++q) child += chain_leaf<D + 1>(L, W, cx + (q & 1) * half, cy + (q >> 1) * half);
-        if (child <= parent) {
-            chain_copy(L, cx, cy, cs, 0, D, false);
-            for (int q = 0; q < 4; ++q) chain_split<D + 1>(L, W, cx + (q & 1) * half, cy + (q >> 1) * half);
-        } else {
-            chain_copy(L, cx, cy, cs, 1, D, false);
+        if (leaf < 0) return;
+        switch (leaf) {
+        case 0: cost = chain_leaf<0>(L, W, lcx, lcy); break;
+        case 1: cost = chain_leaf<1>(L, W, lcx, lcy); break;
+        case 2: cost = chain_leaf<2>(L, W, lcx, lcy); break;
+        default: cost = chain_leaf<3>(L, W, lcx, lcy); break;
         }
     }
 }
 
-__global__ __launch_bounds__(256) void me_chain_kernel(const PicDev *__restrict__ table, ChainArgs a,
+__global__ __launch_bounds__(64) void me_chain_kernel(const PicDev *__restrict__ table, ChainArgs a,
                                                        tcme_chain_call *__restrict__ calls,
                                                        int32_t *__restrict__ counts)
 {
@@ -1138,7 +1219,7 @@ __global__ __launch_bounds__(256) void me_chain_kernel(const PicDev *__restrict_
     const int t = static_cast<int>(threadIdx.x);
     if (t < 64) { L.dx[t] = 0; L.dy[t] = 0; L.un[t] = 1; }
     __syncthreads();
-    chain_split<0>(L, W, 0, 0);
+    chain_walk(L, W);
     if (t == 0) counts[b] = W.nout;
 }
 
@@ -1934,7 +2015,7 @@ hipError_t launch_chain(const PicDev *table, const ChainJobHost *jobs, int njobs
         j.first = jobs[i].first;
         j.cols = jobs[i].cols;
     }
-    hipLaunchKernelGGL(me_chain_kernel, dim3(total_ctus), dim3(256), 0, s, table, a, calls, counts);
+    hipLaunchKernelGGL(me_chain_kernel, dim3(total_ctus), dim3(64), 0, s, table, a, calls, counts);
     return hipGetLastError();
 }
 
