@@ -1259,6 +1259,11 @@ struct LatWin {
     int x0, y0, x1, y1;
 };
 
+// Two signed 16-bit lanes of one dword (v_pk_* integer instructions).
+typedef short pk16 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ pk16 as_pk16(uint32_t v) { return __builtin_bit_cast(pk16, v); }
+__device__ __forceinline__ uint32_t as_u32(pk16 v) { return __builtin_bit_cast(uint32_t, v); }
+
 // The signed 16-bit field of v starting at bit `bit` (0 or 16).  The
 // builtin's result type is unsigned: it is converted before any arithmetic,
 // or sums of negative fields would wrap and shift logically.
@@ -1380,10 +1385,14 @@ struct GroupWalk {
     __device__ __forceinline__ int qsad(const LatWin &lt, int tx, int ty) const
     {
         const int cx = (tx >> 1) - 2 * lt.x0;           // lattice column of X0 at c = 0
-        const int s16 = (((tx + 1) >> 1) - (tx >> 1)) * 16;  // X1 - X0, in bits
         const uint32_t hs = static_cast<uint32_t>(cx & 1) * 16;
+        // byte selectors: the low halves of two dwords (M[.][X0] of two
+        // samples) and the halves holding M[.][X1] (high when X1 = X0 + 1)
+        const uint32_t sel0 = 0x05040100u;
+        const uint32_t sel1 = ((tx + 1) >> 1) != (tx >> 1) ? 0x07060302u : sel0;
         const int y0 = (ty >> 1) - 2 * lt.y0 + 2 * j * R;
         const int y1 = ((ty + 1) >> 1) - 2 * lt.y0 + 2 * j * R;
+        const pk16 one = {1, 1}, lo = {0, 0}, hi = {255, 255};
         uint32_t acc = 0;
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
@@ -1397,18 +1406,30 @@ struct GroupWalk {
                     a[t] = p0[4 * d + t];
                     b[t] = p1[4 * d + t];
                 }
-                uint32_t pk = 0;
+                // sample t's two lattice halves from X0 on: M[.][X0] low,
+                // M[.][X0 + 1] high
+                uint32_t da[4], db[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    // the two halves from X0 on: M[.][X0] low, M[.][X0 + 1] high
-                    const uint32_t da = __builtin_amdgcn_alignbit(a[t + 1], a[t], hs);
-                    const uint32_t db = __builtin_amdgcn_alignbit(b[t + 1], b[t], hs);
-                    const int top = (half16(da, 0) + half16(da, s16) + 1) >> 1;
-                    const int bot = (half16(db, 0) + half16(db, s16) + 1) >> 1;
-                    int v = (top + bot + 1) >> 1;
-                    v = v < 0 ? 0 : (v > 255 ? 255 : v);
-                    pk |= static_cast<uint32_t>(v) << (8 * t);
+                    da[t] = __builtin_amdgcn_alignbit(a[t + 1], a[t], hs);
+                    db[t] = __builtin_amdgcn_alignbit(b[t + 1], b[t], hs);
                 }
+                // two samples per packed 16-bit operation (the values stay in
+                // [-420, 931], so no lane wraps)
+                uint32_t v2[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const pk16 a0 = as_pk16(__builtin_amdgcn_perm(da[2 * h + 1], da[2 * h], sel0));
+                    const pk16 a1 = as_pk16(__builtin_amdgcn_perm(da[2 * h + 1], da[2 * h], sel1));
+                    const pk16 b0 = as_pk16(__builtin_amdgcn_perm(db[2 * h + 1], db[2 * h], sel0));
+                    const pk16 b1 = as_pk16(__builtin_amdgcn_perm(db[2 * h + 1], db[2 * h], sel1));
+                    const pk16 top = (a0 + a1 + one) >> 1;
+                    const pk16 bot = (b0 + b1 + one) >> 1;
+                    const pk16 v = (top + bot + one) >> 1;
+                    v2[h] = as_u32(__builtin_elementwise_min(__builtin_elementwise_max(v, lo), hi));
+                }
+                // the low byte of each of the four 16-bit results
+                const uint32_t pk = __builtin_amdgcn_perm(v2[1], v2[0], 0x06040200u);
                 acc = __builtin_amdgcn_sad_u8(pk, o[rr * DW + d], acc);
                 // keep each group of four samples' loads with their use
                 // (bounds the registers in flight; other wavefronts hide
