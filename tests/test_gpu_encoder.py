@@ -99,25 +99,44 @@ def site_lines(stderr):
 
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("name", list(CASES))
-def test_dropin_encoder_bytes_identical_and_every_call_matches(clips, tmp_path, name):
+def test_dropin_encoder_bytes_identical_and_every_call_matches(clips, tmp_path, capfd, name):
     for b in ("tcenc", "tcenc_gpu", "tcdec", "tcenc_fcheck"):
         need(os.path.join(REF, b))
     yuv, w, h, flags, sites = clips[name]
     args = ["-w", str(w), "-h", str(h), *flags]
     out = {k: str(tmp_path / f"{k}.tcv") for k in ("cpu", "gpu", "chk")}
+    # the check-mode encoder runs verbose (-v: one stderr line per coded
+    # picture, no effect on the bytes), which the test relays as progress
     cmds = {"cpu": [os.path.join(REF, "tcenc"), *args, "-o", out["cpu"], yuv],
             "gpu": [os.path.join(REF, "tcenc_gpu"), *args, "-o", out["gpu"], yuv],
-            "chk": [os.path.join(REF, "tcenc_fcheck"), *args, "-o", out["chk"], yuv]}
+            "chk": [os.path.join(REF, "tcenc_fcheck"), *args, "-v", "-o", out["chk"], yuv]}
     env = {"cpu": None, "gpu": dict(os.environ, TCME_HOOK_STATS="1"),
            "chk": dict(os.environ, TCME_SHIM="check")}
     print(f"{name}: three encoders at once", flush=True)
     sys.stdout.flush()
-    procs = {k: subprocess.Popen(c, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
-                                 env=env[k]) for k, c in cmds.items()}
+    logs = {k: open(tmp_path / f"{k}.err", "w+") for k in ("cpu", "gpu")}
+    procs = {k: subprocess.Popen(cmds[k], stdout=subprocess.DEVNULL, stderr=logs[k], text=True,
+                                 env=env[k]) for k in ("cpu", "gpu")}
+    procs["chk"] = subprocess.Popen(cmds["chk"], stdout=subprocess.DEVNULL,
+                                    stderr=subprocess.PIPE, text=True, env=env["chk"])
+    chk_err = []
+    for ln in procs["chk"].stderr:
+        chk_err.append(ln)
+        if ln.startswith("Frame"):
+            # the test's own output is captured; a coded picture is reported
+            # on the terminal so that a long case shows its progress
+            with capfd.disabled():
+                print(f"{name}: check-mode {ln.strip()}", flush=True)
     res = {}
     for k, p in procs.items():
-        so, se = p.communicate(timeout=850)
-        res[k] = (p.returncode, so, se)
+        p.wait(timeout=850)
+        if k == "chk":
+            se = "".join(chk_err)
+        else:
+            logs[k].seek(0)
+            se = logs[k].read()
+            logs[k].close()
+        res[k] = (p.returncode, "", se)
         print(f"{name}: {k} exit {p.returncode}", flush=True)
     for k, (rc, so, se) in res.items():
         assert rc == 0, f"{k}: {se[-2000:]}"
