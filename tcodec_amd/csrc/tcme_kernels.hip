@@ -631,9 +631,21 @@ struct Searcher {
         }
         const int x0 = bx - 1, y0 = by - 1;  // window origin (sample coordinates)
         constexpr int WN = Gm::WIN * Gm::WIN;
-        for (int i = tid; i < WN; i += 64 * WPC) {
-            const int r = i / Gm::WIN, c = i - r * Gm::WIN;
-            qw[i] = aplane[static_cast<size_t>(y0 + r) * R.apitch + x0 + c];
+        // up to 8 loads per thread in flight before their stores (a rolled
+        // loop waited for each: a 34x34 window is 19 loads per lane, a 66x66
+        // one 9 per thread of 8 wavefronts)
+        constexpr int kStep = 64 * WPC, kBatch = 8;
+        for (int i0 = tid; i0 < WN; i0 += kBatch * kStep) {
+            anc_t v[kBatch];
+#pragma unroll
+            for (int b = 0; b < kBatch; ++b) {
+                const int i = i0 + b * kStep;
+                const int r = i / Gm::WIN, c = i - r * Gm::WIN;
+                if (i < WN) v[b] = aplane[static_cast<size_t>(y0 + r) * R.apitch + x0 + c];
+            }
+#pragma unroll
+            for (int b = 0; b < kBatch; ++b)
+                if (i0 + b * kStep < WN) qw[i0 + b * kStep] = v[b];
         }
         sync_call();
 #pragma unroll 1
@@ -1648,7 +1660,12 @@ __global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile
                                                             tcme_stats *__restrict__ stats)
 {
     __shared__ TileShared<WAVES> sh;
-    const int b = tile_slot(blockIdx.x, gridDim.x);
+    // chunks from the end of the list first: lists in CTU order end with the
+    // picture's partial edge CTUs, whose chunks straddle two CTUs (two runs,
+    // twice the staging), and reference-major lists over a most-recent-first
+    // picture buffer with the oldest references, whose calls walk furthest
+    // and defer most; started last, such chunks set the launch's tail
+    const int b = static_cast<int>(gridDim.x) - 1 - tile_slot(blockIdx.x, gridDim.x);
     const int c0 = b * chunk;
     if (c0 >= n) return;
     const int cnt = n - c0 < chunk ? n - c0 : chunk;
