@@ -1484,10 +1484,12 @@ struct GroupWalk {
 // gives the same results; lists grouped by (CTU, reference), as an encoder
 // issues them, make one run per chunk.
 // Lanes per call by block size: 64x64 one call per wavefront, 32x32 four,
-// 16x16 eight, 8x8 sixteen (each lane then holds 64, 64, 32 or 16 samples of
-// its block).
+// 16x16 and 8x8 eight (each lane then holds 64, 64, 32 or 8 samples of its
+// block: one row of an 8x8 block, so a CTU's 64 8x8 calls are 8 slots with
+// short candidate chains; 16 calls of two rows each per wavefront measured
+// 0.5% slower, 32 of four rows 5% slower).
 template <int N> struct Lanes {
-    static constexpr int L = N == 64 ? 64 : N == 32 ? 16 : N == 16 ? 8 : 4;
+    static constexpr int L = N == 64 ? 64 : N == 32 ? 16 : 8;
     static constexpr int C = 64 / L;  // calls per slot
 };
 __host__ __device__ constexpr int size_class(int n) { return n == 64 ? 0 : n == 32 ? 1 : n == 16 ? 2 : 3; }
@@ -1798,7 +1800,7 @@ __global__ __launch_bounds__(64 * WAVES, BIG && WAVES == 8 ? 4 : 1) void me_tile
             }
         }
         __syncthreads();
-        // slots, largest blocks first: 64 (1 call), 32 (4), 16 (8), 8 (16)
+        // slots, largest blocks first: 64 (1 call), 32 (4), 16 (8), 8 (8)
         const int n64 = sh.nlist[0], n32 = sh.nlist[1], n16 = sh.nlist[2], n8 = sh.nlist[3];
         constexpr int C64 = Lanes<64>::C, C32 = Lanes<32>::C, C16 = Lanes<16>::C, C8 = Lanes<8>::C;
         const int s64 = (n64 + C64 - 1) / C64, s32 = (n32 + C32 - 1) / C32;
