@@ -595,6 +595,35 @@ struct Searcher {
         }
     }
 
+    // SAD of a bilinear quarter-pel candidate (reference src/motion.c:111-116,
+    // 330-336) from the sample fields of the LDS window: ox, oy (0 or 1)
+    // locate its sample origin in the window, dx, dy its fraction.
+    __device__ __forceinline__ int32_t qsad_win_bil(int ox, int oy, int dx, int dy)
+    {
+        const anc_t *w = qw + oy * Gm::WIN + ox;
+        const auto px = [dx, dy](const anc_t *e) {
+            const int top = (a_i(e[0]) * (4 - dx) + a_i(e[1]) * dx + 2) >> 2;
+            const int bot = (a_i(e[Gm::WIN]) * (4 - dx) + a_i(e[Gm::WIN + 1]) * dx + 2) >> 2;
+            return (top * (4 - dy) + bot * dy + 2) >> 2;
+        };
+        if (N == 8) {
+            const int d = o8 - px(w + (lane >> 3) * Gm::WIN + (lane & 7));
+            return wave_sum(d < 0 ? -d : d);
+        }
+        uint32_t acc = 0;
+#pragma unroll
+        for (int u = 0; u < Gm::U; ++u) {
+            const anc_t *e = w + urow[u] * Gm::WIN + ucol[u];
+            const uint32_t pk = static_cast<uint32_t>(px(e)) | (static_cast<uint32_t>(px(e + 1)) << 8) |
+                                (static_cast<uint32_t>(px(e + 2)) << 16) |
+                                (static_cast<uint32_t>(px(e + 3)) << 24);
+            acc = __builtin_amdgcn_sad_u8(pk, o[u], acc);
+        }
+        int32_t v[1] = {wave_sum(static_cast<int>(acc))};
+        block_sum<1>(v);
+        return v[0];
+    }
+
     // Quarter-pel refinement from the planes, every candidate priced with its
     // own class (six-tap or bilinear) straight from the anchor and luma planes.
     __device__ __forceinline__ void qpel_direct(int bx, int by, int &qx, int &qy, int32_t &best)
@@ -614,21 +643,23 @@ struct Searcher {
     }
 
     // Quarter-pel refinement, one greedy pass in the reference's order
-    // (reference src/motion.c:450-481).  (bx,by) is the integer best.  When
-    // every sample position the pass can reach (fx in {bx-1, bx}, fy in
-    // {by-1, by}) is six-tap safe, the block's (N+2)^2 anchor window is staged
-    // in LDS once and every candidate is six-tap; otherwise candidates are
-    // priced from the planes with their own class.
+    // (reference src/motion.c:450-481).  (bx,by) is the integer best.  Every
+    // sample position the pass can reach lies in fx in {bx-1, bx}, fy in
+    // {by-1, by}, so the block's (N+2)^2 anchor window at (bx-1, by-1) is
+    // staged in LDS once (coordinates clamped into the picture: near an edge
+    // the clamped entries are never read) and each candidate is priced from
+    // it with its own class: six-tap from the half samples, bilinear from
+    // the sample fields, or skipped (reference src/motion.c:455-464).
     __device__ __forceinline__ void qpel(int bx, int by, int &qx, int &qy, int32_t &best)
     {
         constexpr int K = Pat<PAT_QPEL>::K;
         qx = bx * 4;
         qy = by * 4;
-        const bool fast = QWIN && bx - 1 >= 2 && by - 1 >= 2 && bx + N + 2 < R.w && by + N + 2 < R.h;
-        if (!fast) {
+        if (!QWIN) {
             qpel_direct(bx, by, qx, qy, best);
             return;
         }
+        const bool six = bx - 1 >= 2 && by - 1 >= 2 && bx + N + 2 < R.w && by + N + 2 < R.h;
         const int x0 = bx - 1, y0 = by - 1;  // window origin (sample coordinates)
         constexpr int WN = Gm::WIN * Gm::WIN;
         // up to 8 loads per thread in flight before their stores (a rolled
@@ -641,7 +672,8 @@ struct Searcher {
             for (int b = 0; b < kBatch; ++b) {
                 const int i = i0 + b * kStep;
                 const int r = i / Gm::WIN, c = i - r * Gm::WIN;
-                if (i < WN) v[b] = aplane[static_cast<size_t>(y0 + r) * R.apitch + x0 + c];
+                const int y = min(max(y0 + r, 0), R.h - 1), x = min(max(x0 + c, 0), R.w - 1);
+                if (i < WN) v[b] = aplane[static_cast<size_t>(y) * R.apitch + x];
             }
 #pragma unroll
             for (int b = 0; b < kBatch; ++b)
@@ -651,8 +683,16 @@ struct Searcher {
 #pragma unroll 1
         for (int k = 0; k < K; ++k) {
             const int tx = qx + Pat<PAT_QPEL>::x(k), ty = qy + Pat<PAT_QPEL>::y(k);
-            cnt.q6++;
-            const int32_t sv = qsad_win_any((tx >> 2) - x0, (ty >> 2) - y0, tx & 3, ty & 3);
+            const int cls = six ? 1 : qclass(R, N, tx, ty);
+            if (!cls) continue;
+            int32_t sv;
+            if (cls == 1) {
+                cnt.q6++;
+                sv = qsad_win_any((tx >> 2) - x0, (ty >> 2) - y0, tx & 3, ty & 3);
+            } else {
+                cnt.qb++;
+                sv = qsad_win_bil((tx >> 2) - x0, (ty >> 2) - y0, tx & 3, ty & 3);
+            }
             if (sv < best) { best = sv; qx = tx; qy = ty; }
         }
     }
