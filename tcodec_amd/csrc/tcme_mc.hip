@@ -5,8 +5,10 @@
 //                     get_interp_luma's per-sample ladder (src/motion.c:269-304)
 //   mc_chroma_kernel  tc_inter_predict_chroma, src/motion.c:597-632
 //
-// One wavefront per block; each XCD works through a contiguous share of the
-// list (mc_share).  Every branch of the reference's luma predictor (row copy
+// Each XCD works through a contiguous share of the list (luma: a chunk per
+// workgroup, runs of 8x8 blocks eight to a wavefront, large blocks split
+// over a workgroup's wavefronts; chroma: mc_share, quads of small blocks per
+// wavefront).  Every branch of the reference's luma predictor (row copy
 // for an in-frame integer MV, whole-block six-tap, per-sample six-tap /
 // bilinear / nearest ladder) gives, sample for sample, the ladder's value, so
 // the kernel computes the
@@ -223,18 +225,91 @@ __device__ __forceinline__ McShare mc_share(int n, int wave)
     return m;
 }
 
-// One wavefront per predicted block; the grid is at most a few workgroups
-// per CU and each wavefront strides over its XCD's share of the list
-// (mc_share; a block is only ~2,000 cycles of work, so one workgroup per four
-// blocks left the dispatcher, not the CUs, setting the pace).  The block
+// Rows [rs, re) of one predicted block by the whole wavefront (the block
 // index is wave-uniform, so the descriptor and the picture-table entry are
-// scalar loads, and the entry is reused while the picture stays the same.
+// scalar loads, and the entry is reused while the picture stays the same).
 // Per-sample selects of the fraction made the sample loop issue-bound, so an
 // interior block - every sample's 6x6 apron inside the frame, where the
 // ladder is the six-tap value, or the sample itself at an integer MV - runs
 // code specialised for its fraction, up to 4 rows per lane at a time, loads
 // first (8 rows took 84 VGPRs, 5 wavefronts per SIMD, and was slower).
-// Other blocks take the ladder sample by sample.
+// Other blocks take the ladder sample by sample.  The status is written by
+// the wavefront that takes row 0.
+__device__ __forceinline__ void luma_rows(const PicDev *__restrict__ table, int npics,
+                                          const tcme_pred &p, int64_t i, int rs, int re,
+                                          uint8_t *__restrict__ out, int64_t out_bytes,
+                                          int32_t *__restrict__ status, PicDev &R, int &cached,
+                                          int lane)
+{
+    const int st = check_pred(table, npics, p, out_bytes, true, R, cached);
+    if (status && lane == 0 && rs == 0) status[i] = st;
+    if (st) return;
+    re = re < p.blk ? re : p.blk;
+    if (rs >= re) return;
+    const int fx = p.mv_x >> 2, fy = p.mv_y >> 2, dx = p.mv_x & 3, dy = p.mv_y & 3;
+    const BlockLanes bl(p.blk, lane);
+    const bool inner = fx >= 2 && fy >= 2 && fx + p.blk + 3 <= R.w && fy + p.blk + 3 <= R.h;
+    if (!inner) {
+        const gout8 o = (gout8)out + p.out_off;
+        for (int r = rs + bl.r0; r < re; r += bl.dr)
+            o[static_cast<int64_t>(r) * p.out_stride + bl.c] =
+                static_cast<uint8_t>(luma_px(R, fx + bl.c, fy + r, dx, dy));
+        return;
+    }
+    const int nr = re - rs;                          // rows of the band
+    const int rows = (nr + bl.dr - 1) / bl.dr;        // per lane; wave-uniform
+    const gout8 oc = (gout8)out + p.out_off + static_cast<int64_t>(rs) * p.out_stride + bl.c;
+    if ((dx | dy) == 0) {
+        const gptr8 y0 = (gptr8)R.y + static_cast<size_t>(fy + rs) * R.pitch + fx + bl.c;
+        if (rows >= 4) mc_copy<4>(y0, R.pitch, oc, p.out_stride, bl.r0, bl.dr, nr);
+        else mc_copy<1>(y0, R.pitch, oc, p.out_stride, bl.r0, bl.dr, nr);
+        return;
+    }
+    const gptr64 a0 = (gptr64)R.an + static_cast<size_t>(fy + rs) * R.apitch + fx + bl.c;
+    if (rows >= 4) mc_interior_k<4>(dx, dy, a0, R.apitch, oc, p.out_stride, bl.r0, bl.dr, nr);
+    else mc_interior_k<1>(dx, dy, a0, R.apitch, oc, p.out_stride, bl.r0, bl.dr, nr);
+}
+
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) u64x2 *gptr64x2;
+
+// Rows [r0, r0 + 8) of column c of interior block p (every sample's six-tap
+// apron inside the frame), the fraction held per lane: the anchor pairs
+// (x, x+1) of the 9 rows are loaded first, then each sample is the lattice
+// formula with the anchor fields selected at run time.
+__device__ __forceinline__ void column8(const PicDev &R, const tcme_pred &p, int c, int r0,
+                                        uint8_t *__restrict__ out)
+{
+    const int fx = p.mv_x >> 2, fy = p.mv_y >> 2, dx = p.mv_x & 3, dy = p.mv_y & 3;
+    const gptr64 a = (gptr64)R.an + static_cast<size_t>(fy + r0) * R.apitch + fx + c;
+    const size_t ap = static_cast<size_t>(R.apitch);
+    u64x2 e[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) e[r] = *(gptr64x2)(a + r * ap);
+    const gout8 o = (gout8)out + p.out_off + static_cast<int64_t>(r0) * p.out_stride + c;
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+        o[static_cast<int64_t>(r) * p.out_stride] =
+            static_cast<uint8_t>(lattice_px(e[r].x, e[r].y, e[r + 1].x, e[r + 1].y, dx, dy));
+}
+
+// Workgroup b (4 wavefronts) takes a contiguous chunk of the list; the
+// chunks of each XCD are contiguous (workgroup b runs on XCD b % 8), so the
+// blocks of one CTU, whose quadtree levels read the same anchors, meet in
+// one L2.  Inside a chunk:
+//  1. wavefront w takes the chunk's groups of 8 consecutive descriptors
+//     w, w + 4, ...: when all 8 are interior 8x8 blocks of one picture (most
+//     of a frame's blocks, in runs of 64 per CTU) it predicts them at once -
+//     lane 8b + c takes column c of block b, its descriptor a per-lane load,
+//     and walks the column down the block (column8): one round trip to
+//     memory per 8 blocks and no scalar work per block (the same for groups
+//     of 16x16 blocks, four per pass, took 87 VGPRs and was slower);
+//     otherwise it predicts the group's blocks of at most 16x16 one by one;
+//  2. the four wavefronts then take the chunk's 32x32 and 64x64 blocks
+//     together, each a quarter of every block's rows (one wavefront alone
+//     on a 64x64 block made 16 dependent rounds of loads, which set the
+//     launch's length).
+// Every wavefront finds the same large blocks, so no barrier is needed.
 __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__ table, int npics,
                                                       const tcme_pred *__restrict__ preds, int n,
                                                       uint8_t *__restrict__ out, int64_t out_bytes,
@@ -242,35 +317,58 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
 {
     const int lane = static_cast<int>(threadIdx.x & 63);
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-    const McShare sh = mc_share(n, wave);
+    constexpr int kXcd = 8;
+    const int g = static_cast<int>(gridDim.x), bid = static_cast<int>(blockIdx.x);
+    const int q = g / kXcd, rr = g % kXcd, x = bid % kXcd;
+    const int chunk = x * q + (x < rr ? x : rr) + bid / kXcd;   // XCD x's chunks are contiguous
+    const int64_t c0 = static_cast<int64_t>(n) * chunk / g;
+    const int64_t c1 = static_cast<int64_t>(n) * (chunk + 1) / g;
     PicDev R{};
     int cached = -1;
-    for (int64_t i = sh.first; i < sh.end; i += sh.step) {
-        const tcme_pred p = preds[i];
-        const int st = check_pred(table, npics, p, out_bytes, true, R, cached);
-        if (status && lane == 0) status[i] = st;
-        if (st) continue;
-        const int fx = p.mv_x >> 2, fy = p.mv_y >> 2, dx = p.mv_x & 3, dy = p.mv_y & 3;
-        const gout8 o = (gout8)out + p.out_off;
-        const BlockLanes bl(p.blk, lane);
-        const bool inner = fx >= 2 && fy >= 2 && fx + p.blk + 3 <= R.w && fy + p.blk + 3 <= R.h;
-        if (!inner) {
-            for (int r = bl.r0; r < p.blk; r += bl.dr)
-                o[static_cast<int64_t>(r) * p.out_stride + bl.c] =
-                    static_cast<uint8_t>(luma_px(R, fx + bl.c, fy + r, dx, dy));
+    const int b = lane >> 3, c = lane & 7;
+    for (int64_t i0 = c0 + 8 * wave; i0 < c1; i0 += 32) {
+        const int cnt = c1 - i0 < 8 ? static_cast<int>(c1 - i0) : 8;
+        tcme_pred p{};
+        if (b < cnt) p = preds[i0 + b];
+        const int pic0 = __builtin_amdgcn_readfirstlane(p.pic);
+        constexpr int nb = 8;
+        bool fast = cnt == 8 && pic0 >= 0 && pic0 < npics &&
+                    __ballot(p.pic != pic0 || p.blk != nb) == 0;
+        if (fast) {
+            if (pic0 != cached) {
+                R = table[pic0];
+                cached = pic0;
+            }
+            const int fx = p.mv_x >> 2, fy = p.mv_y >> 2;
+            const bool ok = R.y && R.an && span_fits(p.out_off, nb, nb, p.out_stride, out_bytes) &&
+                            fx >= 2 && fy >= 2 && fx + nb + 3 <= R.w && fy + nb + 3 <= R.h;
+            fast = __ballot(!ok) == 0;
+        }
+        if (!fast) {
+            for (int k = 0; k < cnt; ++k) {
+                const tcme_pred pk = preds[i0 + k];
+                if (pk.blk < 32 || pk.blk > kMaxBlk)
+                    luma_rows(table, npics, pk, i0 + k, 0, kMaxBlk, out, out_bytes, status, R,
+                              cached, lane);
+            }
             continue;
         }
-        const int rows = (p.blk + bl.dr - 1) / bl.dr;  // per lane; wave-uniform
-        const gout8 oc = o + bl.c;
-        if ((dx | dy) == 0) {
-            const gptr8 y0 = (gptr8)R.y + static_cast<size_t>(fy) * R.pitch + fx + bl.c;
-            if (rows >= 4) mc_copy<4>(y0, R.pitch, oc, p.out_stride, bl.r0, bl.dr, p.blk);
-            else mc_copy<1>(y0, R.pitch, oc, p.out_stride, bl.r0, bl.dr, p.blk);
-            continue;
+        if (status && c == 0) status[i0 + b] = 0;
+        column8(R, p, c, 0, out);   // lane 8b + c: column c of block b
+    }
+    // the chunk's 32x32 and 64x64 blocks, a quarter of the rows per wavefront
+    for (int64_t j0 = c0; j0 < c1; j0 += 64) {
+        const int64_t j = j0 + lane;
+        const bool big = j < c1 && preds[j].blk >= 32 && preds[j].blk <= kMaxBlk;
+        uint64_t m = __ballot(big);
+        while (m) {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1;
+            const tcme_pred pk = preds[j0 + k];
+            const int band = (pk.blk + 3) / 4;  // luma_rows clips the last band
+            luma_rows(table, npics, pk, j0 + k, wave * band, (wave + 1) * band, out, out_bytes,
+                      status, R, cached, lane);
         }
-        const gptr64 a0 = (gptr64)R.an + static_cast<size_t>(fy) * R.apitch + fx + bl.c;
-        if (rows >= 4) mc_interior_k<4>(dx, dy, a0, R.apitch, oc, p.out_stride, bl.r0, bl.dr, p.blk);
-        else mc_interior_k<1>(dx, dy, a0, R.apitch, oc, p.out_stride, bl.r0, bl.dr, p.blk);
     }
 }
 
@@ -369,7 +467,8 @@ hipError_t launch_predict(bool chroma, const PicDev *table, int npics, const tcm
                           int n, uint8_t *out, int64_t out_bytes, int32_t *status, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    const dim3 grid(std::min((n + 3) / 4, 8 * device_cus()));
+    // luma: a chunk of the list per workgroup (at least one descriptor)
+    const dim3 grid(std::min(chroma ? (n + 3) / 4 : n, 8 * device_cus()));
     if (chroma)
         hipLaunchKernelGGL(mc_chroma_kernel, grid, dim3(256), 0, s, table, npics, preds, n, out,
                            out_bytes, status);

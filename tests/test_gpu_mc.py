@@ -43,32 +43,34 @@ def test_gpu_luma_mc_matches_oracle(engine, orc, shape):
     engine.free(rp)
 
 
-def test_gpu_luma_mc_8x8_quads(engine, orc):
-    """Runs of 8x8 blocks (as a frame's call list holds them), predicted four
-    to a wavefront: every phase, integer MVs, quads whose blocks are all in
-    the interior, quads with one block on or past the interior's edge (the
-    whole quad then takes the per-block path), lists whose length is not a
-    multiple of four, and a rejected descriptor inside a quad."""
+@pytest.mark.parametrize("n", [8, 16])
+def test_gpu_luma_mc_runs_of_small_blocks(engine, orc, n):
+    """Runs of 8x8 or 16x16 blocks (as a frame's call list holds them),
+    predicted eight to a wavefront: every phase, integer MVs, groups whose
+    blocks are all in the interior, groups with one block on or past the
+    interior's edge (the whole group then takes the per-block path), lists
+    whose length is not a multiple of eight, and a rejected descriptor inside
+    a group."""
     from tcodec_amd import me, synth
-    print("luma mc 8x8 quads", flush=True)
+    print(f"luma mc runs of {n}x{n} blocks", flush=True)
     h, w = 240, 320
     ref = synth.Scene(w, h, seed=12, rects=10).frame(1)[0]
     rp = engine.pic(ref)
-    rng = np.random.default_rng(8)
+    rng = np.random.default_rng(8 + n)
     for count in (4001, 4000, 6):
-        sizes = np.full(count, 8)
-        fx = rng.integers(2, w - 8 - 2, count)
-        fy = rng.integers(2, h - 8 - 2, count)
+        sizes = np.full(count, n)
+        fx = rng.integers(2, w - n - 2, count)
+        fy = rng.integers(2, h - n - 2, count)
         mvs = np.stack([fx * 4 + rng.integers(0, 4, count), fy * 4 + rng.integers(0, 4, count)], 1)
         mvs[::7] &= ~3                                        # integer MVs
         if count > 100:
-            # one block per some quads at the interior's edge or beyond it
+            # one block of some groups at the interior's edge or beyond it
             for k in range(1, count, 37):
-                mvs[k] = [(w - 8 - 3 + int(rng.integers(0, 3))) * 4 + int(rng.integers(0, 4)),
+                mvs[k] = [(w - n - 3 + int(rng.integers(0, 3))) * 4 + int(rng.integers(0, 4)),
                           int(rng.integers(-40, 8 * 4))]
         preds = _preds(rp, mvs, sizes)
         if count > 100:
-            preds["pic"][50] = 999                            # rejected inside a quad
+            preds["pic"][50] = 999                            # rejected inside a group
         out, st = engine.predict(preds)
         for k in range(len(preds)):
             if count > 100 and k == 50:
@@ -76,8 +78,8 @@ def test_gpu_luma_mc_8x8_quads(engine, orc):
                 continue
             assert st[k] == 0
             o = int(preds["out_off"][k])
-            got = out[o:o + 64].reshape(8, 8)
-            want = orc.inter_predict(ref, int(mvs[k, 0]), int(mvs[k, 1]), 8)
+            got = out[o:o + n * n].reshape(n, n)
+            want = orc.inter_predict(ref, int(mvs[k, 0]), int(mvs[k, 1]), n)
             assert np.array_equal(got, want), (count, k, mvs[k])
     engine.free(rp)
 
@@ -93,7 +95,7 @@ def test_gpu_luma_mc_interior_blocks(engine, orc):
     rp = engine.pic(ref)
     rng = np.random.default_rng(5)
     count = 3000
-    sizes = rng.choice([8, 16, 32, 64, 12, 24, 48], count)
+    sizes = rng.choice([8, 16, 32, 64, 12, 24, 48, 37, 50, 63], count)
     fx = np.array([rng.integers(2, w - n - 2) for n in sizes])
     fy = np.array([rng.integers(2, h - n - 2) for n in sizes])
     mvs = np.stack([fx * 4 + rng.integers(0, 4, count), fy * 4 + rng.integers(0, 4, count)], 1)
