@@ -273,43 +273,62 @@ __device__ __forceinline__ void luma_rows(const PicDev *__restrict__ table, int 
 typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 typedef const __attribute__((address_space(1))) u64x2 *gptr64x2;
 
-// Rows [r0, r0 + 8) of column c of interior block p (every sample's six-tap
-// apron inside the frame), the fraction held per lane: the anchor pairs
-// (x, x+1) of the 9 rows are loaded first, then each sample is the lattice
-// formula with the anchor fields selected at run time.
-__device__ __forceinline__ void column8(const PicDev &R, const tcme_pred &p, int c, int r0,
-                                        uint8_t *__restrict__ out)
+__device__ __forceinline__ int shfl32(int v, int src)
 {
-    const int fx = p.mv_x >> 2, fy = p.mv_y >> 2, dx = p.mv_x & 3, dy = p.mv_y & 3;
-    const gptr64 a = (gptr64)R.an + static_cast<size_t>(fy + r0) * R.apitch + fx + c;
-    const size_t ap = static_cast<size_t>(R.apitch);
+    return __builtin_amdgcn_ds_bpermute(src << 2, v);
+}
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
+{
+    const uint32_t lo = static_cast<uint32_t>(shfl32(static_cast<int>(v), src));
+    const uint32_t hi = static_cast<uint32_t>(shfl32(static_cast<int>(v >> 32), src));
+    return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+}
+
+// Rows [r0, r0 + 8) of column c of an interior block (every sample's six-tap
+// apron inside the frame) at quarter-pel (mvx, mvy), anchors an / apitch,
+// output o (the block's first row) with row stride os: the anchor pairs
+// (x, x+1) of the 9 rows are loaded first, then each sample is the lattice
+// formula with the anchor fields selected at run time (the fraction is held
+// per lane).
+__device__ __forceinline__ void column8(const uint64_t *an, int apitch, int mvx, int mvy, int c,
+                                        int r0, uint8_t *o, int os)
+{
+    const int fx = mvx >> 2, fy = mvy >> 2, dx = mvx & 3, dy = mvy & 3;
+    const gptr64 a = (gptr64)an + static_cast<size_t>(fy + r0) * apitch + fx + c;
+    const size_t ap = static_cast<size_t>(apitch);
     u64x2 e[9];
 #pragma unroll
     for (int r = 0; r < 9; ++r) e[r] = *(gptr64x2)(a + r * ap);
-    const gout8 o = (gout8)out + p.out_off + static_cast<int64_t>(r0) * p.out_stride + c;
+    const gout8 oc = (gout8)o + static_cast<int64_t>(r0) * os + c;
 #pragma unroll
     for (int r = 0; r < 8; ++r)
-        o[static_cast<int64_t>(r) * p.out_stride] =
+        oc[static_cast<int64_t>(r) * os] =
             static_cast<uint8_t>(lattice_px(e[r].x, e[r].y, e[r + 1].x, e[r + 1].y, dx, dy));
 }
 
 // Workgroup b (4 wavefronts) takes a contiguous chunk of the list; the
 // chunks of each XCD are contiguous (workgroup b runs on XCD b % 8), so the
 // blocks of one CTU, whose quadtree levels read the same anchors, meet in
-// one L2.  Inside a chunk:
-//  1. wavefront w takes the chunk's groups of 8 consecutive descriptors
-//     w, w + 4, ...: when all 8 are interior 8x8 blocks of one picture (most
-//     of a frame's blocks, in runs of 64 per CTU) it predicts them at once -
-//     lane 8b + c takes column c of block b, its descriptor a per-lane load,
-//     and walks the column down the block (column8): one round trip to
-//     memory per 8 blocks and no scalar work per block (the same for groups
-//     of 16x16 blocks, four per pass, took 87 VGPRs and was slower);
-//     otherwise it predicts the group's blocks of at most 16x16 one by one;
-//  2. the four wavefronts then take the chunk's 32x32 and 64x64 blocks
-//     together, each a quarter of every block's rows (one wavefront alone
-//     on a 64x64 block made 16 dependent rounds of loads, which set the
-//     launch's length).
-// Every wavefront finds the same large blocks, so no barrier is needed.
+// one L2.  The chunk is taken 64 descriptors at a time, lane k of every
+// wavefront holding descriptor k:
+//  * interior 8x8 and 16x16 blocks (every sample's six-tap apron inside the
+//    frame) are cut into tasks of one column x 8 rows (8 per 8x8 block, 32
+//    per 16x16); the tasks of the 64 descriptors are numbered in a row
+//    (prefix sum over the lanes) and the workgroup's 256 lanes take 256 at a
+//    time, each finding its block by a binary search over the prefix sums
+//    and the block's fields by lane permutes, then loading its 9 anchor rows
+//    at once (column8): a pass is one round trip to memory and every
+//    wavefront gets the same share (one wavefront per block spent 2-4
+//    dependent rounds of loads on each);
+//  * 32x32 and 64x64 blocks are shared by the four wavefronts, a quarter of
+//    the rows each, with code specialised for the block's fraction (one
+//    wavefront alone on a 64x64 block made 16 dependent rounds of loads; as
+//    column tasks, with the fraction selected per sample, they cost 2.5x
+//    more);
+//  * other blocks (edges, other sizes, rejected descriptors) are predicted
+//    one per wavefront, in turn.
+// Every wavefront computes the same prefix sums and lists, so no barrier is
+// needed.
 __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__ table, int npics,
                                                       const tcme_pred *__restrict__ preds, int n,
                                                       uint8_t *__restrict__ out, int64_t out_bytes,
@@ -325,49 +344,74 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
     const int64_t c1 = static_cast<int64_t>(n) * (chunk + 1) / g;
     PicDev R{};
     int cached = -1;
-    const int b = lane >> 3, c = lane & 7;
-    for (int64_t i0 = c0 + 8 * wave; i0 < c1; i0 += 32) {
-        const int cnt = c1 - i0 < 8 ? static_cast<int>(c1 - i0) : 8;
+    for (int64_t w0 = c0; w0 < c1; w0 += 64) {
+        const int cnt = c1 - w0 < 64 ? static_cast<int>(c1 - w0) : 64;
         tcme_pred p{};
-        if (b < cnt) p = preds[i0 + b];
-        const int pic0 = __builtin_amdgcn_readfirstlane(p.pic);
-        constexpr int nb = 8;
-        bool fast = cnt == 8 && pic0 >= 0 && pic0 < npics &&
-                    __ballot(p.pic != pic0 || p.blk != nb) == 0;
-        if (fast) {
-            if (pic0 != cached) {
-                R = table[pic0];
-                cached = pic0;
+        bool ok = false;
+        uint64_t an = 0;
+        int apitch = 0, lg = 0;
+        if (lane < cnt) {
+            p = preds[w0 + lane];
+            lg = p.blk == 8 ? 3 : p.blk == 16 ? 4 : 0;
+            if (lg && p.pic >= 0 && p.pic < npics) {
+                const PicDev Rk = table[p.pic];
+                const int fx = p.mv_x >> 2, fy = p.mv_y >> 2;
+                ok = Rk.y && Rk.an && span_fits(p.out_off, p.blk, p.blk, p.out_stride, out_bytes) &&
+                     fx >= 2 && fy >= 2 && fx + p.blk + 3 <= Rk.w && fy + p.blk + 3 <= Rk.h;
+                an = reinterpret_cast<uint64_t>(Rk.an);
+                apitch = Rk.apitch;
             }
-            const int fx = p.mv_x >> 2, fy = p.mv_y >> 2;
-            const bool ok = R.y && R.an && span_fits(p.out_off, nb, nb, p.out_stride, out_bytes) &&
-                            fx >= 2 && fy >= 2 && fx + nb + 3 <= R.w && fy + nb + 3 <= R.h;
-            fast = __ballot(!ok) == 0;
         }
-        if (!fast) {
-            for (int k = 0; k < cnt; ++k) {
-                const tcme_pred pk = preds[i0 + k];
-                if (pk.blk < 32 || pk.blk > kMaxBlk)
-                    luma_rows(table, npics, pk, i0 + k, 0, kMaxBlk, out, out_bytes, status, R,
-                              cached, lane);
+        const int tasks = ok ? p.blk * p.blk / 8 : 0;
+        int incl = tasks;  // inclusive prefix sum over the lanes
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = shfl32(incl, lane >= d ? lane - d : lane);
+            if (lane >= d) incl += t;
+        }
+        const int excl = incl - tasks;
+        const int total = __builtin_amdgcn_readlane(incl, 63);
+        if (status && ok && wave == 0) status[w0 + lane] = 0;
+        for (int t0 = wave * 64; t0 < total; t0 += 256) {
+            // lanes past the last task take the last task's block and rows
+            // (so every address is one of an eligible block) and store nothing
+            const int t = t0 + lane, tt = t < total ? t : total - 1;
+            // the last descriptor whose tasks start at or before tt (excl is
+            // non-decreasing; descriptors without tasks share the next one's
+            // start and are passed over)
+            int k = 0;
+#pragma unroll
+            for (int sft = 32; sft; sft >>= 1) {
+                const int e = shfl32(excl, k + sft < 64 ? k + sft : 63);
+                if (k + sft < cnt && e <= tt) k += sft;
             }
-            continue;
+            const int u = tt - shfl32(excl, k), kl = shfl32(lg, k);
+            const int mvx = shfl32(p.mv_x, k), mvy = shfl32(p.mv_y, k);
+            const int os = shfl32(p.out_stride, k), ap = shfl32(apitch, k);
+            const uint64_t a = shfl64(an, k), oo = shfl64(static_cast<uint64_t>(p.out_off), k);
+            if (t < total)
+                column8(reinterpret_cast<const uint64_t *>(a), ap, mvx, mvy, u & ((1 << kl) - 1),
+                        (u >> kl) * 8, out + oo, os);
         }
-        if (status && c == 0) status[i0 + b] = 0;
-        column8(R, p, c, 0, out);   // lane 8b + c: column c of block b
-    }
-    // the chunk's 32x32 and 64x64 blocks, a quarter of the rows per wavefront
-    for (int64_t j0 = c0; j0 < c1; j0 += 64) {
-        const int64_t j = j0 + lane;
-        const bool big = j < c1 && preds[j].blk >= 32 && preds[j].blk <= kMaxBlk;
-        uint64_t m = __ballot(big);
+        // 32x32 and 64x64 blocks: every wavefront a quarter of the rows
+        uint64_t m = __ballot(lane < cnt && p.blk >= 32 && p.blk <= kMaxBlk);
         while (m) {
             const int k = __builtin_ctzll(m);
             m &= m - 1;
-            const tcme_pred pk = preds[j0 + k];
+            const tcme_pred pk = preds[w0 + k];
             const int band = (pk.blk + 3) / 4;  // luma_rows clips the last band
-            luma_rows(table, npics, pk, j0 + k, wave * band, (wave + 1) * band, out, out_bytes,
+            luma_rows(table, npics, pk, w0 + k, wave * band, (wave + 1) * band, out, out_bytes,
                       status, R, cached, lane);
+        }
+        // the rest (edges, other sizes, rejected descriptors), one
+        // descriptor per wavefront in turn
+        m = __ballot(lane < cnt && !ok && (p.blk < 32 || p.blk > kMaxBlk));
+        for (int j = 0; m; ++j) {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1;
+            if ((j & 3) == wave)
+                luma_rows(table, npics, preds[w0 + k], w0 + k, 0, kMaxBlk, out, out_bytes, status,
+                          R, cached, lane);
         }
     }
 }
