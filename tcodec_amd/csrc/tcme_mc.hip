@@ -5,10 +5,10 @@
 //                     get_interp_luma's per-sample ladder (src/motion.c:269-304)
 //   mc_chroma_kernel  tc_inter_predict_chroma, src/motion.c:597-632
 //
-// Each XCD works through a contiguous share of the list (luma: a chunk per
-// workgroup, runs of 8x8 blocks eight to a wavefront, large blocks split
-// over a workgroup's wavefronts; chroma: mc_share, quads of small blocks per
-// wavefront).  Every branch of the reference's luma predictor (row copy
+// Each XCD works through a contiguous share of the list, a chunk per
+// workgroup: small interior blocks are cut into column tasks that the
+// workgroup's lanes share evenly, large luma blocks into row bands, one per
+// wavefront.  Every branch of the reference's luma predictor (row copy
 // for an in-frame integer MV, whole-block six-tap, per-sample six-tap /
 // bilinear / nearest ladder) gives, sample for sample, the ladder's value, so
 // the kernel computes the
@@ -194,35 +194,6 @@ __device__ __forceinline__ void mc_interior_k(int dx, int dy, gptr64 a0, int ap,
 #undef TCME_MC_CASE
     default: break;
     }
-}
-
-// Block list -> wavefronts: each XCD (workgroup b runs on XCD b % 8) takes a
-// contiguous share of the list in proportion to its wavefronts, so the blocks
-// of one CTU, which read overlapping anchors, meet in one L2; the XCD's
-// wavefronts stride over its share, neighbouring wavefronts on neighbouring
-// blocks.  Every block of [0, n) falls to exactly one wavefront.  (Against
-// all wavefronts striding over the whole list: 103 us against 115 us for the
-// 171,600 blocks of a 1080p frame; runs of 4 consecutive blocks per
-// wavefront balanced worse, 120 us.)
-struct McShare {
-    int64_t first;  // this wavefront's first block
-    int64_t end;    // end of this XCD's share
-    int64_t step;   // wavefronts on this XCD
-};
-
-__device__ __forceinline__ McShare mc_share(int n, int wave)
-{
-    constexpr int kXcd = 8;
-    const int g = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x), x = b % kXcd;
-    const int q = g / kXcd, r = g % kXcd;
-    const int64_t waves = 4LL * g;
-    const int64_t before = 4LL * (x * q + min(x, r));    // wavefronts of XCDs < x
-    const int64_t mine = 4LL * (q + (x < r ? 1 : 0));    // wavefronts of XCD x
-    McShare m;
-    m.first = static_cast<int64_t>(n) * before / waves + static_cast<int64_t>(b / kXcd) * 4 + wave;
-    m.end = static_cast<int64_t>(n) * (before + mine) / waves;
-    m.step = mine;
-    return m;
 }
 
 // Rows [rs, re) of one predicted block by the whole wavefront (the block
@@ -448,12 +419,59 @@ __device__ __forceinline__ void chroma_rows(const PicDev &R, int xi, int yi, int
     }
 }
 
-// Chroma at 1/8 sample (tc_inter_predict_chroma, src/motion.c:597-632).  The
-// list is taken in quads of consecutive blocks (mc_share over quads): when all
-// four are at most 4x4 (a 4:2:0 8x8 luma node's chroma, most of a frame's
-// blocks), each takes 16 lanes of the wavefront, its descriptor loaded per
-// lane; otherwise the wavefront predicts them one after another, up to 4
-// rows per lane at a time.
+// One chroma block by the whole wavefront, up to 4 rows per lane at a time
+// (the descriptor index is wave-uniform: scalar descriptor and table loads).
+__device__ __forceinline__ void chroma_block(const PicDev *__restrict__ table, int npics,
+                                             const tcme_pred &p, int64_t i,
+                                             uint8_t *__restrict__ out, int64_t out_bytes,
+                                             int32_t *__restrict__ status, PicDev &R, int &cached,
+                                             int lane)
+{
+    const int st = check_pred(table, npics, p, out_bytes, false, R, cached);
+    if (status && lane == 0) status[i] = st;
+    if (st) return;
+    const int xi = p.mv_x >> 3, yi = p.mv_y >> 3, fx = p.mv_x & 7, fy = p.mv_y & 7;
+    const gout8 o = (gout8)out + p.out_off;
+    const BlockLanes bl(p.blk, lane);
+    if ((p.blk + bl.dr - 1) / bl.dr >= 4)  // rows per lane; wave-uniform
+        chroma_rows<4>(R, xi, yi, fx, fy, o, p.out_stride, bl.c, bl.r0, bl.dr, p.blk);
+    else
+        chroma_rows<1>(R, xi, yi, fx, fy, o, p.out_stride, bl.c, bl.r0, bl.dr, p.blk);
+}
+
+// Rows [r0, r0 + 4) of column c of a chroma block whose every sample's 2x2
+// neighbourhood is inside the plane (no clamping), plane y / pitch, at
+// 1/8-sample (mvx, mvy): the two samples (x, x+1) of the 5 rows are loaded
+// first.
+__device__ __forceinline__ void chroma_col4(const uint8_t *y, int pitch, int mvx, int mvy, int c,
+                                            int r0, uint8_t *o, int os)
+{
+    const int xi = mvx >> 3, yi = mvy >> 3, fx = mvx & 7, fy = mvy & 7;
+    const gptr8 p = (gptr8)y + static_cast<size_t>(yi + r0) * pitch + xi + c;
+    int a[5], b[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        a[r] = p[static_cast<size_t>(r) * pitch];
+        b[r] = p[static_cast<size_t>(r) * pitch + 1];
+    }
+    const int w00 = (8 - fx) * (8 - fy), w10 = fx * (8 - fy), w01 = (8 - fx) * fy, w11 = fx * fy;
+    const gout8 oc = (gout8)o + static_cast<int64_t>(r0) * os + c;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        oc[static_cast<int64_t>(r) * os] = static_cast<uint8_t>(
+            (w00 * a[r] + w10 * b[r] + w01 * a[r + 1] + w11 * b[r + 1] + 32) >> 6);
+}
+
+// Chroma at 1/8 sample (tc_inter_predict_chroma, src/motion.c:597-632), the
+// same work split as the luma kernel: a chunk of the list per workgroup,
+// taken 64 descriptors at a time; blocks of 4x4 .. 32x32 whose samples need
+// no edge clamping are cut into tasks of one column x 4 rows (m * m / 4 per
+// block), numbered in a row over the 64 descriptors and taken by the
+// workgroup's 256 lanes 256 at a time (chroma_col4: one round trip to memory
+// per pass); other blocks are predicted one per wavefront, in turn.  (Quads
+// of consecutive blocks per wavefront, four 4x4 blocks at 16 lanes each and
+// larger ones one after another, took 63 us for a 1080p frame's 343,200
+// blocks, most of it in the 16x16 and 32x32 blocks' rounds of loads.)
 __global__ __launch_bounds__(256) void mc_chroma_kernel(const PicDev *__restrict__ table,
                                                         int npics,
                                                         const tcme_pred *__restrict__ preds,
@@ -463,44 +481,65 @@ __global__ __launch_bounds__(256) void mc_chroma_kernel(const PicDev *__restrict
 {
     const int lane = static_cast<int>(threadIdx.x & 63);
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-    const McShare sh = mc_share((n + 3) / 4, wave);
+    constexpr int kXcd = 8;
+    const int g = static_cast<int>(gridDim.x), bid = static_cast<int>(blockIdx.x);
+    const int q = g / kXcd, rr = g % kXcd, x = bid % kXcd;
+    const int chunk = x * q + (x < rr ? x : rr) + bid / kXcd;   // XCD x's chunks are contiguous
+    const int64_t c0 = static_cast<int64_t>(n) * chunk / g;
+    const int64_t c1 = static_cast<int64_t>(n) * (chunk + 1) / g;
     PicDev R{};
     int cached = -1;
-    for (int64_t q = sh.first; q < sh.end; q += sh.step) {
-        const int64_t i0 = q * 4;
-        const int cnt = n - i0 < 4 ? static_cast<int>(n - i0) : 4;
-        bool small = cnt == 4;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (small) small = preds[i0 + k].blk >= 1 && preds[i0 + k].blk <= 4;
-        if (small) {
-            const int g = lane >> 4, l = lane & 15;
-            const tcme_pred p = preds[i0 + g];
-            PicDev Rg{};
-            int none = -1;
-            const int st = check_pred(table, npics, p, out_bytes, false, Rg, none);
-            if (status && l == 0) status[i0 + g] = st;
-            if (st == 0 && l < p.blk * p.blk) {
-                const int r = l / p.blk, c = l - r * p.blk;
-                const gout8 o = (gout8)out + p.out_off;
-                o[static_cast<int64_t>(r) * p.out_stride + c] = static_cast<uint8_t>(
-                    chroma_px(Rg, p.mv_x >> 3, p.mv_y >> 3, p.mv_x & 7, p.mv_y & 7, c, r));
+    for (int64_t w0 = c0; w0 < c1; w0 += 64) {
+        const int cnt = c1 - w0 < 64 ? static_cast<int>(c1 - w0) : 64;
+        tcme_pred p{};
+        bool ok = false;
+        uint64_t y = 0;
+        int pitch = 0, lg = 0;
+        if (lane < cnt) {
+            p = preds[w0 + lane];
+            lg = p.blk == 4 ? 2 : p.blk == 8 ? 3 : p.blk == 16 ? 4 : p.blk == 32 ? 5 : 0;
+            if (lg && p.pic >= 0 && p.pic < npics) {
+                const PicDev Rk = table[p.pic];
+                const int xi = p.mv_x >> 3, yi = p.mv_y >> 3;
+                ok = Rk.y && span_fits(p.out_off, p.blk, p.blk, p.out_stride, out_bytes) &&
+                     xi >= 0 && yi >= 0 && xi + p.blk + 1 <= Rk.w && yi + p.blk + 1 <= Rk.h;
+                y = reinterpret_cast<uint64_t>(Rk.y);
+                pitch = Rk.pitch;
             }
-            continue;
         }
-        for (int k = 0; k < cnt; ++k) {
-            const int64_t i = i0 + k;
-            const tcme_pred p = preds[i];
-            const int st = check_pred(table, npics, p, out_bytes, false, R, cached);
-            if (status && lane == 0) status[i] = st;
-            if (st) continue;
-            const int xi = p.mv_x >> 3, yi = p.mv_y >> 3, fx = p.mv_x & 7, fy = p.mv_y & 7;
-            const gout8 o = (gout8)out + p.out_off;
-            const BlockLanes bl(p.blk, lane);
-            if ((p.blk + bl.dr - 1) / bl.dr >= 4)  // rows per lane; wave-uniform
-                chroma_rows<4>(R, xi, yi, fx, fy, o, p.out_stride, bl.c, bl.r0, bl.dr, p.blk);
-            else
-                chroma_rows<1>(R, xi, yi, fx, fy, o, p.out_stride, bl.c, bl.r0, bl.dr, p.blk);
+        const int tasks = ok ? p.blk * p.blk / 4 : 0;
+        int incl = tasks;  // inclusive prefix sum over the lanes
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = shfl32(incl, lane >= d ? lane - d : lane);
+            if (lane >= d) incl += t;
+        }
+        const int excl = incl - tasks;
+        const int total = __builtin_amdgcn_readlane(incl, 63);
+        if (status && ok && wave == 0) status[w0 + lane] = 0;
+        for (int t0 = wave * 64; t0 < total; t0 += 256) {
+            const int t = t0 + lane, tt = t < total ? t : total - 1;
+            int k = 0;  // the last descriptor whose tasks start at or before tt
+#pragma unroll
+            for (int sft = 32; sft; sft >>= 1) {
+                const int e = shfl32(excl, k + sft < 64 ? k + sft : 63);
+                if (k + sft < cnt && e <= tt) k += sft;
+            }
+            const int u = tt - shfl32(excl, k), kl = shfl32(lg, k);
+            const int mvx = shfl32(p.mv_x, k), mvy = shfl32(p.mv_y, k);
+            const int os = shfl32(p.out_stride, k), pt = shfl32(pitch, k);
+            const uint64_t yy = shfl64(y, k), oo = shfl64(static_cast<uint64_t>(p.out_off), k);
+            if (t < total)
+                chroma_col4(reinterpret_cast<const uint8_t *>(yy), pt, mvx, mvy,
+                            u & ((1 << kl) - 1), (u >> kl) * 4, out + oo, os);
+        }
+        uint64_t m = __ballot(lane < cnt && !ok);
+        for (int j = 0; m; ++j) {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1;
+            if ((j & 3) == wave)
+                chroma_block(table, npics, preds[w0 + k], w0 + k, out, out_bytes, status, R,
+                             cached, lane);
         }
     }
 }
@@ -511,8 +550,8 @@ hipError_t launch_predict(bool chroma, const PicDev *table, int npics, const tcm
                           int n, uint8_t *out, int64_t out_bytes, int32_t *status, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    // luma: a chunk of the list per workgroup (at least one descriptor)
-    const dim3 grid(std::min(chroma ? (n + 3) / 4 : n, 8 * device_cus()));
+    // a chunk of the list per workgroup (at least one descriptor)
+    const dim3 grid(std::min(n, 8 * device_cus()));
     if (chroma)
         hipLaunchKernelGGL(mc_chroma_kernel, grid, dim3(256), 0, s, table, npics, preds, n, out,
                            out_bytes, status);

@@ -197,11 +197,11 @@ def test_gpu_chroma_mc_matches_oracle(engine, orc, shape):
     engine.free(cp)
 
 
-def test_gpu_chroma_mc_small_block_quads(engine, orc):
-    """Runs of 1x1..4x4 blocks (four consecutive ones share a wavefront)
-    between larger blocks, quads that mix the two, a rejected descriptor inside
-    a run of small blocks, and a list whose length is not a multiple of 4."""
-    print("chroma mc small-block quads", flush=True)
+def test_gpu_chroma_mc_small_blocks_near_edges(engine, orc):
+    """Runs of 1x1..4x4 blocks between larger ones on a small plane (most of
+    them clamped at an edge, so predicted one per wavefront), a rejected
+    descriptor inside a run, and a list whose length is not a multiple of 4."""
+    print("chroma mc small blocks near edges", flush=True)
     h, w = 40, 64
     pl, _ = moving_pair(h, w, 9)
     cp = engine.plane(pl)
@@ -210,7 +210,7 @@ def test_gpu_chroma_mc_small_block_quads(engine, orc):
     mvs = np.stack([rng.integers(-80, 8 * w + 80, len(sizes)),
                     rng.integers(-80, 8 * h + 80, len(sizes))], 1)
     preds = _preds(cp, mvs, sizes)
-    preds["pic"][2] = 999      # unknown picture, inside the first quad of 4x4 blocks
+    preds["pic"][2] = 999      # unknown picture, inside the first run of 4x4 blocks
     out, st = engine.predict(preds, chroma=True)
     assert st[2] == -4 and (np.delete(st, 2) == 0).all()
     for k in range(len(preds)):
@@ -240,3 +240,41 @@ def test_gpu_mc_strided_output_and_bad_descriptors(engine, orc):
         assert np.array_equal(got, orc.inter_predict(ref, int(mvs[k, 0]), int(mvs[k, 1]), n))
     engine.free(rp)
     engine.free(pp)
+
+
+def test_gpu_chroma_mc_interior_column_tasks(engine, orc):
+    """Blocks of 4x4 .. 32x32 whose every sample's 2x2 neighbourhood is inside
+    the plane (cut into column tasks shared by a workgroup's lanes), every
+    1/8-sample fraction, in runs of one size and mixed as a CTU's quadtree
+    levels are, next to blocks one sample past that region and other sizes,
+    with a rejected descriptor among them."""
+    from tcodec_amd import synth
+    print("chroma mc interior column tasks 320x240", flush=True)
+    h, w = 240, 320
+    pl = synth.Scene(w, h, seed=21, rects=9).frame(2)[1]
+    pl = np.ascontiguousarray(np.repeat(np.repeat(pl, 2, 0), 2, 1)[:h, :w])
+    cp = engine.plane(pl)
+    rng = np.random.default_rng(77)
+    sizes = np.concatenate([np.full(300, 4), np.full(70, 8), np.full(20, 16), np.full(9, 32),
+                            np.tile([32, 16, 16, 16, 16, 8, 8, 8, 8, 4, 4, 4, 4], 40),
+                            rng.choice([1, 2, 3, 5, 12, 20], 60)])
+    count = len(sizes)
+    xi = np.array([rng.integers(0, w - m) for m in sizes])
+    yi = np.array([rng.integers(0, h - m) for m in sizes])
+    mvs = np.stack([xi * 8 + rng.integers(0, 8, count), yi * 8 + rng.integers(0, 8, count)], 1)
+    # every fraction on the first 64 blocks; some one sample past the region
+    mvs[:64] = np.stack([xi[:64] * 8 + np.arange(64) % 8, yi[:64] * 8 + np.arange(64) // 8], 1)
+    for k in range(5, count, 41):
+        mvs[k, 0] = (w - sizes[k]) * 8 + int(rng.integers(0, 8))
+    preds = _preds(cp, mvs, sizes)
+    preds["pic"][100] = 999
+    out, st = engine.predict(preds, chroma=True)
+    assert st[100] == -4 and (np.delete(st, 100) == 0).all()
+    for k in range(count):
+        if k == 100:
+            continue
+        n = int(sizes[k])
+        o = int(preds["out_off"][k])
+        want = orc.inter_predict_chroma(pl, int(mvs[k, 0]), int(mvs[k, 1]), n)
+        assert np.array_equal(out[o:o + n * n].reshape(n, n), want), (k, mvs[k], n)
+    engine.free(cp)
