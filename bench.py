@@ -471,7 +471,26 @@ def encoder_leg(reps: int = 3):
     return out
 
 
-def mc_leg(eng, frames, d_res, scene=None, pics=None, reps: int = 5):
+def device_ms(fn, reps: int, s) -> float:
+    """Average device time of fn() (launches on stream s) over reps calls:
+    one untimed call, then a spin kernel that keeps the GPU busy while the
+    host enqueues the timed calls, so host launch overhead (e.g. a busy host
+    after the CPU baseline) does not leave gaps in the timed region."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(s):
+        torch.cuda._sleep(20_000_000)
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def mc_leg(eng, frames, d_res, scene=None, pics=None, reps: int = 20):
     """Luma motion compensation of every block of the first frame at its own
     search result (the consumer of each search): one launch, blocks packed;
     then, with the scene, both 4:2:0 chroma planes of every block at the same
@@ -491,16 +510,8 @@ def mc_leg(eng, frames, d_res, scene=None, pics=None, reps: int = 5):
     d_p = torch.from_numpy(preds.view(np.uint8).copy()).cuda()
     d_o = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
     s = torch.cuda.current_stream()
-    eng.predict_device(d_p.data_ptr(), len(preds), d_o.data_ptr(), nbytes, None, False,
-                       s.cuda_stream)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(reps):
-        eng.predict_device(d_p.data_ptr(), len(preds), d_o.data_ptr(), nbytes, None, False,
-                           s.cuda_stream)
-    e1.record(s)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
+    ms = device_ms(lambda: eng.predict_device(d_p.data_ptr(), len(preds), d_o.data_ptr(), nbytes,
+                                              None, False, s.cuda_stream), reps, s)
     algo = int(((n + 5) ** 2 + n * n).sum())   # reference apron read + block written
     out = {"kernel": "mc_luma_kernel", "blocks": len(preds), "pixels": nbytes,
            "ms": round(ms, 3), "Gpixels_per_s": round(nbytes / ms / 1e6, 2),
@@ -525,15 +536,8 @@ def mc_leg(eng, frames, d_res, scene=None, pics=None, reps: int = 5):
         cbytes = int((mm * mm).sum())
         d_cp = torch.from_numpy(cp.view(np.uint8).copy()).cuda()
         d_co = torch.empty(cbytes, dtype=torch.uint8, device="cuda")
-        eng.predict_device(d_cp.data_ptr(), len(cp), d_co.data_ptr(), cbytes, None, True,
-                           s.cuda_stream)
-        e0.record(s)
-        for _ in range(reps):
-            eng.predict_device(d_cp.data_ptr(), len(cp), d_co.data_ptr(), cbytes, None, True,
-                               s.cuda_stream)
-        e1.record(s)
-        torch.cuda.synchronize()
-        cms = e0.elapsed_time(e1) / reps
+        cms = device_ms(lambda: eng.predict_device(d_cp.data_ptr(), len(cp), d_co.data_ptr(),
+                                                   cbytes, None, True, s.cuda_stream), reps, s)
         calgo = int(((mm + 1) ** 2 + mm * mm).sum())   # bilinear apron read + block written
         out["chroma"] = {"kernel": "mc_chroma_kernel", "blocks": len(cp), "pixels": cbytes,
                          "ms": round(cms, 3), "Gpixels_per_s": round(cbytes / cms / 1e6, 2),
@@ -586,14 +590,7 @@ def chain_leg(eng, pics, reps: int = 3):
 
     out = {}
     for name, fn in (("one_launch", all_in_one), ("launch_per_picture", per_picture)):
-        fn()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(s)
-        for _ in range(reps):
-            fn()
-        e1.record(s)
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
+        ms = device_ms(fn, reps, s)
         out[name] = {"ms_per_8_pictures": round(ms, 3),
                      "frames_per_s": round(len(rec) / ms * 1e3, 1)}
     calls = int(d_counts.sum().item())
@@ -639,14 +636,7 @@ def aux_leg(eng, frames, pics, reps: int = 3):
 
     out = {}
     for name, fn in (("global_mv", gmv), ("stability_sad", sads), ("anchors", anchors)):
-        fn()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(s)
-        for _ in range(reps):
-            fn()
-        e1.record(s)
-        torch.cuda.synchronize()
-        out[name] = round(e0.elapsed_time(e1) / reps, 4)
+        out[name] = round(device_ms(fn, reps, s), 4)
     assert (d_out[:, 2] == 0).all().item() and (d_s >= 0).all().item()
     res = {"global_mv": {"pairs": len(pairs), "ms": out["global_mv"],
                          "ms_per_pair": round(out["global_mv"] / len(pairs), 3)},
