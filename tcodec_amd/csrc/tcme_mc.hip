@@ -5,21 +5,20 @@
 //                     get_interp_luma's per-sample ladder (src/motion.c:269-304)
 //   mc_chroma_kernel  tc_inter_predict_chroma, src/motion.c:597-632
 //
-// Each XCD works through a contiguous share of the list, a chunk per
-// workgroup: small interior blocks are cut into column tasks that the
-// workgroup's lanes share evenly, large luma blocks into row bands, one per
-// wavefront.  Every branch of the reference's luma predictor (row copy
-// for an in-frame integer MV, whole-block six-tap, per-sample six-tap /
-// bilinear / nearest ladder) gives, sample for sample, the ladder's value, so
-// the kernel computes the
-// ladder directly: six-tap where the sample's 6x6 apron is inside the frame,
-// bilinear where its 2x2 neighbourhood is, nearest (clamped) otherwise.  The
-// six-tap value comes from the anchor plane (sample, hh, vv, cc per position)
-// read as the half-sample lattice M[Y][X]: with X0 = q >> 1, X1 = (q+1) >> 1
-// per axis at quarter-pel q,
+// Every branch of the reference's luma predictor (row copy for an in-frame
+// integer MV, whole-block six-tap, per-sample six-tap / bilinear / nearest
+// ladder) gives, sample for sample, the ladder's value, so the luma kernel
+// computes the ladder directly: six-tap where the sample's 6x6 window is
+// inside the frame, bilinear where its 2x2 neighbourhood is, nearest
+// (clamped) otherwise.  With the half-sample lattice M[Y][X] (M[2y][2x] the
+// sample, M[2y][2x+1] the raw horizontal six-tap hh, M[2y+1][2x] the raw
+// vertical one vv, M[2y+1][2x+1] the diagonal cc) and X0 = q >> 1,
+// X1 = (q+1) >> 1 per axis at quarter-pel q, the six-tap value is
 //   v = clip((avg(M[Y0][X0], M[Y0][X1]) + avg(M[Y1][X0], M[Y1][X1]) + 1) >> 1),
 // avg(a,b) = (a+b+1) >> 1 on raw values (src/motion.c:193-260; the h- and
-// v-only paths of src/motion.c:133-191 agree after their final clip).
+// v-only paths of src/motion.c:133-191 agree after their final clip).  The
+// luma kernel computes the lattice values it needs from the luma bytes,
+// staged in LDS per block; the chroma kernel reads the plane directly.
 #include <algorithm>
 
 #include "tcme_internal.h"
@@ -62,33 +61,6 @@ __device__ __forceinline__ int lattice_px(uint64_t a00, uint64_t a10, uint64_t a
     return clip255((top + bot + 1) >> 1);
 }
 
-// get_interp_luma (src/motion.c:269-304) at integer (x,y), fraction (dx,dy):
-// a sample reads its own anchor, and its right / lower neighbours only at
-// phase 3.
-__device__ __forceinline__ int luma_px(const PicDev &R, int x, int y, int dx, int dy)
-{
-    const gptr8 Y = (gptr8)R.y;
-    if (x >= 2 && x + 3 < R.w && y >= 2 && y + 3 < R.h) {
-        if (dx == 0 && dy == 0) return Y[static_cast<size_t>(y) * R.pitch + x];
-        const gptr64 an = (gptr64)R.an + static_cast<size_t>(y) * R.apitch + x;
-        const uint64_t a00 = an[0];
-        const uint64_t a10 = dx == 3 ? an[1] : a00;
-        const uint64_t a01 = dy == 3 ? an[R.apitch] : a00;
-        const uint64_t a11 = dx == 3 && dy == 3 ? an[R.apitch + 1] : a00;
-        return lattice_px(a00, a10, a01, a11, dx, dy);
-    }
-    if (x >= 0 && y >= 0 && x + 1 < R.w && y + 1 < R.h) {
-        const gptr8 p = Y + static_cast<size_t>(y) * R.pitch + x;
-        const int top = lerp4(p[0], p[1], dx);
-        const int bot = lerp4(p[R.pitch], p[R.pitch + 1], dx);
-        return lerp4(top, bot, dy);
-    }
-    int nx = x + (dx >= 2 ? 1 : 0), ny = y + (dy >= 2 ? 1 : 0);
-    nx = nx < 0 ? 0 : (nx > R.w - 1 ? R.w - 1 : nx);
-    ny = ny < 0 ? 0 : (ny > R.h - 1 ? R.h - 1 : ny);
-    return Y[static_cast<size_t>(ny) * R.pitch + nx];
-}
-
 // Validates a descriptor against the picture table and the output buffer.
 // R holds the table entry of picture `cached` (-1: none) and is reloaded only
 // when the descriptor names another picture (the table does not change
@@ -119,131 +91,6 @@ struct BlockLanes {
     }
 };
 
-// lattice_px at a fraction known at compile time: each field is a fixed
-// half of one dword, and only the neighbours phase 3 needs are read.
-template <int DX, int DY>
-__device__ __forceinline__ int lattice_fixed(uint64_t a00, uint64_t a10, uint64_t a01,
-                                             uint64_t a11)
-{
-    constexpr int X0 = DX >> 1, X1 = (DX + 1) >> 1, Y0 = DY >> 1, Y1 = (DY + 1) >> 1;
-    constexpr int f00 = (Y0 & 1) * 2 + (X0 & 1), f01 = (Y0 & 1) * 2 + (X1 & 1);
-    constexpr int f10 = (Y1 & 1) * 2 + (X0 & 1), f11 = (Y1 & 1) * 2 + (X1 & 1);
-    const uint64_t r0c1 = X1 == 2 ? a10 : a00;
-    const uint64_t r1c0 = Y1 == 2 ? a01 : a00;
-    const uint64_t r1c1 = Y1 == 2 ? (X1 == 2 ? a11 : a01) : (X1 == 2 ? a10 : a00);
-    const int top = (field(a00, f00) + field(r0c1, f01) + 1) >> 1;
-    const int bot = (field(r1c0, f10) + field(r1c1, f11) + 1) >> 1;
-    return clip255((top + bot + 1) >> 1);
-}
-
-// Rows r0, r0 + dr, ... < n of column (lane) of an interior block at fraction
-// (DX,DY), K rows per lane at a time, loads first.  a0 points at the anchor of
-// the lane's sample in the block's first row, o at its output.  Rows past the
-// block re-read its last row and are not stored.
-template <int DX, int DY, int K>
-__device__ __forceinline__ void mc_interior(gptr64 a0, int ap, gout8 o, int ostride, int r0,
-                                            int dr, int n)
-{
-    for (int rb = r0; rb < n; rb += K * dr) {
-        uint64_t a00[K], a10[K], a01[K], a11[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int r = min(rb + k * dr, n - 1);
-            const gptr64 e = a0 + static_cast<size_t>(r) * ap;
-            a00[k] = e[0];
-            a10[k] = DX == 3 ? e[1] : 0;
-            a01[k] = DY == 3 ? e[ap] : 0;
-            a11[k] = DX == 3 && DY == 3 ? e[ap + 1] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int r = rb + k * dr;
-            if (r < n)
-                o[static_cast<int64_t>(r) * ostride] =
-                    static_cast<uint8_t>(lattice_fixed<DX, DY>(a00[k], a10[k], a01[k], a11[k]));
-        }
-    }
-}
-
-// The same for an integer MV: the samples themselves.
-template <int K>
-__device__ __forceinline__ void mc_copy(gptr8 y0, int pitch, gout8 o, int ostride, int r0, int dr,
-                                        int n)
-{
-    for (int rb = r0; rb < n; rb += K * dr) {
-        uint8_t v[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = y0[static_cast<size_t>(min(rb + k * dr, n - 1)) * pitch];
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (rb + k * dr < n) o[static_cast<int64_t>(rb + k * dr) * ostride] = v[k];
-    }
-}
-
-template <int K>
-__device__ __forceinline__ void mc_interior_k(int dx, int dy, gptr64 a0, int ap, gout8 o,
-                                              int ostride, int r0, int dr, int n)
-{
-    switch ((dy << 2) | dx) {
-#define TCME_MC_CASE(DX, DY) \
-    case ((DY) << 2) | (DX): mc_interior<DX, DY, K>(a0, ap, o, ostride, r0, dr, n); break;
-    TCME_MC_CASE(1, 0) TCME_MC_CASE(2, 0) TCME_MC_CASE(3, 0)
-    TCME_MC_CASE(0, 1) TCME_MC_CASE(1, 1) TCME_MC_CASE(2, 1) TCME_MC_CASE(3, 1)
-    TCME_MC_CASE(0, 2) TCME_MC_CASE(1, 2) TCME_MC_CASE(2, 2) TCME_MC_CASE(3, 2)
-    TCME_MC_CASE(0, 3) TCME_MC_CASE(1, 3) TCME_MC_CASE(2, 3) TCME_MC_CASE(3, 3)
-#undef TCME_MC_CASE
-    default: break;
-    }
-}
-
-// Rows [rs, re) of one predicted block by the whole wavefront (the block
-// index is wave-uniform, so the descriptor and the picture-table entry are
-// scalar loads, and the entry is reused while the picture stays the same).
-// Per-sample selects of the fraction made the sample loop issue-bound, so an
-// interior block - every sample's 6x6 apron inside the frame, where the
-// ladder is the six-tap value, or the sample itself at an integer MV - runs
-// code specialised for its fraction, up to 4 rows per lane at a time, loads
-// first (8 rows took 84 VGPRs, 5 wavefronts per SIMD, and was slower).
-// Other blocks take the ladder sample by sample.  The status is written by
-// the wavefront that takes row 0.
-__device__ __forceinline__ void luma_rows(const PicDev *__restrict__ table, int npics,
-                                          const tcme_pred &p, int64_t i, int rs, int re,
-                                          uint8_t *__restrict__ out, int64_t out_bytes,
-                                          int32_t *__restrict__ status, PicDev &R, int &cached,
-                                          int lane)
-{
-    const int st = check_pred(table, npics, p, out_bytes, true, R, cached);
-    if (status && lane == 0 && rs == 0) status[i] = st;
-    if (st) return;
-    re = re < p.blk ? re : p.blk;
-    if (rs >= re) return;
-    const int fx = p.mv_x >> 2, fy = p.mv_y >> 2, dx = p.mv_x & 3, dy = p.mv_y & 3;
-    const BlockLanes bl(p.blk, lane);
-    const bool inner = fx >= 2 && fy >= 2 && fx + p.blk + 3 <= R.w && fy + p.blk + 3 <= R.h;
-    if (!inner) {
-        const gout8 o = (gout8)out + p.out_off;
-        for (int r = rs + bl.r0; r < re; r += bl.dr)
-            o[static_cast<int64_t>(r) * p.out_stride + bl.c] =
-                static_cast<uint8_t>(luma_px(R, fx + bl.c, fy + r, dx, dy));
-        return;
-    }
-    const int nr = re - rs;                          // rows of the band
-    const int rows = (nr + bl.dr - 1) / bl.dr;        // per lane; wave-uniform
-    const gout8 oc = (gout8)out + p.out_off + static_cast<int64_t>(rs) * p.out_stride + bl.c;
-    if ((dx | dy) == 0) {
-        const gptr8 y0 = (gptr8)R.y + static_cast<size_t>(fy + rs) * R.pitch + fx + bl.c;
-        if (rows >= 4) mc_copy<4>(y0, R.pitch, oc, p.out_stride, bl.r0, bl.dr, nr);
-        else mc_copy<1>(y0, R.pitch, oc, p.out_stride, bl.r0, bl.dr, nr);
-        return;
-    }
-    const gptr64 a0 = (gptr64)R.an + static_cast<size_t>(fy + rs) * R.apitch + fx + bl.c;
-    if (rows >= 4) mc_interior_k<4>(dx, dy, a0, R.apitch, oc, p.out_stride, bl.r0, bl.dr, nr);
-    else mc_interior_k<1>(dx, dy, a0, R.apitch, oc, p.out_stride, bl.r0, bl.dr, nr);
-}
-
-typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
-typedef const __attribute__((address_space(1))) u64x2 *gptr64x2;
-
 __device__ __forceinline__ int shfl32(int v, int src)
 {
     return __builtin_amdgcn_ds_bpermute(src << 2, v);
@@ -255,135 +102,427 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
     return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
 }
 
-// Rows [r0, r0 + 8) of column c of an interior block (every sample's six-tap
-// apron inside the frame) at quarter-pel (mvx, mvy), anchors an / apitch,
-// output o (the block's first row) with row stride os: the anchor pairs
-// (x, x+1) of the 9 rows are loaded first, then each sample is the lattice
-// formula with the anchor fields selected at run time (the fraction is held
-// per lane).
-__device__ __forceinline__ void column8(const uint64_t *an, int apitch, int mvx, int mvy, int c,
-                                        int r0, uint8_t *o, int os)
+// Two signed 16-bit lanes of one dword (v_pk_* integer instructions).
+typedef short pk16 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ pk16 as_pk16(uint32_t v) { return __builtin_bit_cast(pk16, v); }
+__device__ __forceinline__ uint32_t as_u32(pk16 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// ---- luma: aprons of luma bytes staged in LDS ------------------------------
+
+constexpr int kMcChunk = 64;      // descriptors per workgroup
+constexpr int kMcWaveLds = 3328;  // bytes of staged aprons per wavefront (>= every slot's)
+
+typedef unsigned int u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef const __attribute__((address_space(1))) u32x4a4 *gptr128u;
+typedef unsigned int u32x4l __attribute__((ext_vector_type(4)));
+
+// A slot: G blocks of one size N (64x64 blocks: a band of 16 rows each),
+// L lanes each, one task (2 columns x 8 rows) per lane.  The apron of a
+// band, rows r0-2 .. r0+BR+2 of the block's reference region and columns
+// from (fx-2) rounded down to a multiple of 4, takes ROWS rows of P bytes
+// (N+5 samples plus up to 3 of alignment, in 16-byte segments).
+template <int N>
+struct McShape {
+    static constexpr int BR = N == 64 ? 16 : N;  // block rows per slot
+    static constexpr int P = 16 * ((N + 8 + 15) / 16);
+    static constexpr int P16 = P / 16;
+    static constexpr int ROWS = BR + 5;
+    static constexpr int PAIRS = N / 2, BANDS = BR / 8;
+    static constexpr int L = PAIRS * BANDS;      // lanes per block band
+    static constexpr int G = 64 / L;             // block bands per slot
+    static constexpr int S = ROWS * P16;         // segments per apron
+    static constexpr int SPL = (S + L - 1) / L;  // segments per lane
+    static_assert(L <= 64 && 64 % L == 0, "lanes");
+    static_assert(G * ROWS * P <= kMcWaveLds, "slot does not fit");
+};
+
+// Per-descriptor facts a slot needs, in LDS (48 bytes).
+struct McInfo {
+    uint64_t src;     // interior: the apron's first byte (aligned down to 4); else the plane's
+    uint64_t dst;     // the block's first output byte
+    int32_t pitch;    // plane pitch
+    int32_t frac;     // dx | dy << 2 | ((fx - 2) & 3) << 4
+    int32_t ostride;  // output row stride
+    int32_t pic;      // picture id
+    int32_t fx, fy;   // integer part of the MV
+    int32_t w, h;     // picture size
+};
+
+// The raw six-tap (1, -5, 20, 20, -5, 1) of six 16-bit values, (+16) >> 5,
+// two at a time (the sums stay within [-6710, 14886]: no lane wraps).
+__device__ __forceinline__ pk16 v6(pk16 a0, pk16 a1, pk16 a2, pk16 a3, pk16 a4, pk16 a5)
 {
-    const int fx = mvx >> 2, fy = mvy >> 2, dx = mvx & 3, dy = mvy & 3;
-    const gptr64 a = (gptr64)an + static_cast<size_t>(fy + r0) * apitch + fx + c;
-    const size_t ap = static_cast<size_t>(apitch);
-    u64x2 e[9];
-#pragma unroll
-    for (int r = 0; r < 9; ++r) e[r] = *(gptr64x2)(a + r * ap);
-    const gout8 oc = (gout8)o + static_cast<int64_t>(r0) * os + c;
-#pragma unroll
-    for (int r = 0; r < 8; ++r)
-        oc[static_cast<int64_t>(r) * os] =
-            static_cast<uint8_t>(lattice_px(e[r].x, e[r].y, e[r + 1].x, e[r + 1].y, dx, dy));
+    const pk16 c16 = {16, 16}, c5 = {5, 5}, c20 = {20, 20};
+    return ((a0 + a5) - c5 * (a1 + a4) + c20 * (a2 + a3) + c16) >> 5;
 }
 
-// Workgroup b (4 wavefronts) takes a contiguous chunk of the list; the
-// chunks of each XCD are contiguous (workgroup b runs on XCD b % 8), so the
-// blocks of one CTU, whose quadtree levels read the same anchors, meet in
-// one L2.  The chunk is taken 64 descriptors at a time, lane k of every
-// wavefront holding descriptor k:
-//  * interior 8x8 and 16x16 blocks (every sample's six-tap apron inside the
-//    frame) are cut into tasks of one column x 8 rows (8 per 8x8 block, 32
-//    per 16x16); the tasks of the 64 descriptors are numbered in a row
-//    (prefix sum over the lanes) and the workgroup's 256 lanes take 256 at a
-//    time, each finding its block by a binary search over the prefix sums
-//    and the block's fields by lane permutes, then loading its 9 anchor rows
-//    at once (column8): a pass is one round trip to memory and every
-//    wavefront gets the same share (one wavefront per block spent 2-4
-//    dependent rounds of loads on each);
-//  * 32x32 and 64x64 blocks are shared by the four wavefronts, a quarter of
-//    the rows each, with code specialised for the block's fraction (one
-//    wavefront alone on a 64x64 block made 16 dependent rounds of loads; as
-//    column tasks, with the fraction selected per sample, they cost 2.5x
-//    more);
-//  * other blocks (edges, other sizes, rejected descriptors) are predicted
-//    one per wavefront, in turn.
-// Every wavefront computes the same prefix sums and lists, so no barrier is
-// needed.
+// get_interp_luma's ladder at one sample with every load issued up front:
+// anchors and samples at the four clamped corners and the nearest sample
+// (each branch reads only coordinates that are in the frame, where the
+// clamped ones equal its own).
+__device__ __forceinline__ int luma_px_flat(const PicDev &R, int x, int y, int dx, int dy)
+{
+    const int cx0 = min(max(x, 0), R.w - 1), cx1 = min(max(x + 1, 0), R.w - 1);
+    const int cy0 = min(max(y, 0), R.h - 1), cy1 = min(max(y + 1, 0), R.h - 1);
+    const int nx = min(max(x + (dx >= 2 ? 1 : 0), 0), R.w - 1);
+    const int ny = min(max(y + (dy >= 2 ? 1 : 0), 0), R.h - 1);
+    const gptr8 Y = (gptr8)R.y;
+    const gptr64 A = (gptr64)R.an;
+    const size_t r0 = static_cast<size_t>(cy0), r1 = static_cast<size_t>(cy1);
+    const uint64_t a00 = A[r0 * R.apitch + cx0], a10 = A[r0 * R.apitch + cx1];
+    const uint64_t a01 = A[r1 * R.apitch + cx0], a11 = A[r1 * R.apitch + cx1];
+    const int p00 = Y[r0 * R.pitch + cx0], p10 = Y[r0 * R.pitch + cx1];
+    const int p01 = Y[r1 * R.pitch + cx0], p11 = Y[r1 * R.pitch + cx1];
+    const int pn = Y[static_cast<size_t>(ny) * R.pitch + nx];
+    if (x >= 2 && x + 3 < R.w && y >= 2 && y + 3 < R.h)
+        return (dx | dy) == 0 ? p00 : lattice_px(a00, a10, a01, a11, dx, dy);
+    if (x >= 0 && y >= 0 && x + 1 < R.w && y + 1 < R.h)
+        return lerp4(lerp4(p00, p10, dx), lerp4(p01, p11, dx), dy);
+    return pn;
+}
+
+// Rows r0 .. r0+7 of columns c, c+1 of a staged block (apron at lds, row
+// pitch P, apron column 0 at byte a of each row) at fraction (dx, dy), the
+// two columns in the 16-bit halves of res[r].  Each of the task's 13 apron
+// rows gives s(x), s(x+1), s(x+2) and the raw hh(x), hh(x+1) (signed-byte
+// dot products: bytes b-128 against the taps, 128*32 added back); the row's
+// lattice values at X0 and X1 (s or hh, by dx) are e0 and e1.  The odd
+// lattice row between rows y and y+1 holds the vertical six-taps of those
+// columns (vv of a sample column, cc of an hh column), so with hE(y) =
+// avg(e0, e1) on row y and hO(y) = avg(v6(e0), v6(e1)) the value is
+// clip(avg(top, bot)) with top = hE(y) or hO(y) and bot = hE(y), hO(y) or
+// hE(y+1), by dy.
+template <int P>
+__device__ __forceinline__ void mc_task(const uint8_t *lds, int a, int dx, int dy, int c, int r0,
+                                        uint32_t (&res)[8])
+{
+    const int off = a + c, sh = off & 3;
+    const uint8_t *b = lds + r0 * P + (off & ~3);
+    constexpr uint32_t kM = 0x80808080u;
+    const int X0 = dx >> 1, X1 = (dx + 1) >> 1;
+    const int K0 = static_cast<int>(0x1414FB01u), K1 = 0x000001FB;
+    const bool xa = X0 != 0, xb0 = X1 == 0, xb2 = X1 == 2;
+    uint32_t E0[13], E1[13];
+#pragma unroll
+    for (int i = 0; i < 13; ++i) {
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(b + i * P);
+        const uint32_t d0 = w[0], d1 = w[1], d2 = w[2];
+        const uint32_t w0 = __builtin_amdgcn_alignbyte(d1, d0, sh);
+        const uint32_t w1 = __builtin_amdgcn_alignbyte(d2, d1, sh);
+        const uint32_t u0 = w0 ^ kM, u1 = w1 ^ kM;
+        const int h0 = __builtin_amdgcn_sdot4(static_cast<int>(u0), K0,
+                                              __builtin_amdgcn_sdot4(static_cast<int>(u1), K1, 4112, false), false);
+        const int h1 = __builtin_amdgcn_sdot4(static_cast<int>(__builtin_amdgcn_alignbyte(u1, u0, 1)), K0,
+                                              __builtin_amdgcn_sdot4(static_cast<int>(u1 >> 8), K1, 4112, false),
+                                              false);
+        // (column c, column c+1) in the two 16-bit halves
+        const uint32_t sp = __builtin_amdgcn_perm(w1, w0, 0x0c030c02u);   // s(x), s(x+1)
+        const uint32_t sq = __builtin_amdgcn_perm(w1, w0, 0x0c040c03u);   // s(x+1), s(x+2)
+        const uint32_t hp = as_u32(as_pk16(__builtin_amdgcn_perm(static_cast<uint32_t>(h1),
+                                                                 static_cast<uint32_t>(h0), 0x05040100u)) >> 5);
+        E0[i] = xa ? hp : sp;
+        E1[i] = xb0 ? sp : (xb2 ? sq : hp);
+    }
+    const bool yt = dy >= 2, y1e = dy == 0, y1f = dy == 3;
+    const pk16 one = {1, 1}, lo = {0, 0}, hi = {255, 255};
+    pk16 hE = (as_pk16(E0[2]) + as_pk16(E1[2]) + one) >> 1;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const pk16 o0 = v6(as_pk16(E0[r]), as_pk16(E0[r + 1]), as_pk16(E0[r + 2]), as_pk16(E0[r + 3]),
+                           as_pk16(E0[r + 4]), as_pk16(E0[r + 5]));
+        const pk16 o1 = v6(as_pk16(E1[r]), as_pk16(E1[r + 1]), as_pk16(E1[r + 2]), as_pk16(E1[r + 3]),
+                           as_pk16(E1[r + 4]), as_pk16(E1[r + 5]));
+        const pk16 hO = (o0 + o1 + one) >> 1;
+        const pk16 hF = (as_pk16(E0[r + 3]) + as_pk16(E1[r + 3]) + one) >> 1;
+        const pk16 top = yt ? hO : hE;
+        const pk16 bot = y1e ? hE : (y1f ? hF : hO);
+        const pk16 v = __builtin_elementwise_min(__builtin_elementwise_max((top + bot + one) >> 1, lo), hi);
+        res[r] = as_u32(v);
+        hE = hF;
+    }
+}
+
+// One slot: up to G blocks of size N (descriptor numbers idx[0..cnt) of the
+// workgroup's chunk; for N = 64 the band of 16 rows `band`), L lanes each,
+// one task per lane.  The lanes stage their block's apron in the
+// wavefront's own LDS region, then compute their task from it.
+//   !EDGE: the apron lies inside the plane: 16-byte loads.
+//   EDGE:  it may not: dword loads with the row clamped into the picture and
+//          the column into the pitch (a clamped dword lies wholly outside the
+//          picture, as the columns are dword-aligned, so every in-picture
+//          byte of the copy is exact); a sample whose 6x6 window is in the
+//          picture keeps the six-tap value, any other one takes the
+//          ladder's bilinear or nearest value from the copy (from the plane
+//          when the nearest sample lies outside the copy).
+template <int N, bool EDGE>
+__device__ __forceinline__ void mc_slot(const McInfo *info, const uint8_t *idx, int cnt, int band,
+                                        uint8_t *lds, int lane)
+{
+    using S = McShape<N>;
+    const int g = lane / S::L, li = lane % S::L;
+    const bool act = g < cnt;
+    const McInfo m = info[idx[act ? g : 0]];
+    uint8_t *blds = lds + g * S::ROWS * S::P;
+    const int rb = band * S::BR;  // the band's first block row
+    if constexpr (!EDGE) {
+        const gptr8 src = (gptr8)m.src + static_cast<size_t>(rb) * m.pitch;
+        // every lane loads (segments past the apron re-read its last one,
+        // the groups without a block read the slot's first block), so the
+        // loads are issued together
+        u32x4l v[S::SPL];
+#pragma unroll
+        for (int i = 0; i < S::SPL; ++i) {
+            const int s = min(li + S::L * i, S::S - 1);
+            const int row = s / S::P16, q = s % S::P16;
+            const u32x4a4 t = *(gptr128u)(src + static_cast<size_t>(row) * m.pitch + 16 * q);
+            v[i] = u32x4l{t.x, t.y, t.z, t.w};
+        }
+#pragma unroll
+        for (int i = 0; i < S::SPL; ++i) {
+            const int s = min(li + S::L * i, S::S - 1);
+            const int row = s / S::P16, q = s % S::P16;
+            *reinterpret_cast<u32x4l *>(blds + row * S::P + 16 * q) = v[i];
+        }
+    } else {
+        constexpr int D4 = S::P / 4, DS = S::ROWS * D4, DPL = (DS + S::L - 1) / S::L;
+        const gptr8 src = (gptr8)m.src;
+        const int xs = (m.fx - 2) & ~3;
+        uint32_t v[DPL];
+#pragma unroll
+        for (int i = 0; i < DPL; ++i) {
+            const int s = min(li + S::L * i, DS - 1);
+            const int row = s / D4, dc = s % D4;
+            const int yy = min(max(m.fy - 2 + rb + row, 0), m.h - 1);
+            const int xx = min(max(xs + 4 * dc, 0), m.pitch - 4);
+            v[i] = *(const __attribute__((address_space(1))) uint32_t *)(
+                src + static_cast<size_t>(yy) * m.pitch + xx);
+        }
+#pragma unroll
+        for (int i = 0; i < DPL; ++i) {
+            const int s = min(li + S::L * i, DS - 1);
+            const int row = s / D4, dc = s % D4;
+            *reinterpret_cast<uint32_t *>(blds + row * S::P + 4 * dc) = v[i];
+        }
+    }
+    // the wavefront's LDS writes before its reads (one wavefront owns the region)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (act) {
+        const int dx = m.frac & 3, dy = (m.frac >> 2) & 3, a = (m.frac >> 4) & 3;
+        const int c = 2 * (li % S::PAIRS), r0 = 8 * (li / S::PAIRS);
+        uint32_t res[8];
+        mc_task<S::P>(blds, a, dx, dy, c, r0, res);
+        const gout8 oc = (gout8)m.dst + static_cast<int64_t>(rb + r0) * m.ostride + c;
+        if constexpr (!EDGE) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                oc[static_cast<int64_t>(r) * m.ostride] = static_cast<uint8_t>(res[r]);
+                oc[static_cast<int64_t>(r) * m.ostride + 1] = static_cast<uint8_t>(res[r] >> 16);
+            }
+        } else {
+            // the six-tap value holds where the sample's 6x6 window is in the
+            // picture; elsewhere the ladder's bilinear (2x2 in the picture)
+            // or nearest sample, read from the LDS copy (in-picture bytes
+            // are exact there), or from the plane when the nearest sample
+            // lies outside the copy
+            const int xs = (m.fx - 2) & ~3, ys = m.fy - 2 + rb;
+#pragma unroll 1
+            for (int k = 0; k < 16; ++k) {
+                const int r = k >> 1, cc = k & 1;
+                const int x = m.fx + c + cc, y = m.fy + rb + r0 + r;
+                int v = static_cast<int>((res[r] >> (16 * cc)) & 0xff);
+                if (!(x >= 2 && x + 3 < m.w && y >= 2 && y + 3 < m.h)) {
+                    if (x >= 0 && y >= 0 && x + 1 < m.w && y + 1 < m.h) {
+                        const uint8_t *p = blds + (y - ys) * S::P + (x - xs);
+                        v = lerp4(lerp4(p[0], p[1], dx), lerp4(p[S::P], p[S::P + 1], dx), dy);
+                    } else {
+                        const int nx = min(max(x + (dx >= 2 ? 1 : 0), 0), m.w - 1);
+                        const int ny = min(max(y + (dy >= 2 ? 1 : 0), 0), m.h - 1);
+                        const int X = nx - xs, Y = ny - ys;
+                        v = X >= 0 && X < S::P && Y >= 0 && Y < S::ROWS
+                                ? blds[Y * S::P + X]
+                                : ((gptr8)m.src)[static_cast<size_t>(ny) * m.pitch + nx];
+                    }
+                }
+                oc[static_cast<int64_t>(r) * m.ostride + cc] = static_cast<uint8_t>(v);
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Workgroup b (4 wavefronts) takes kMcChunk consecutive descriptors (the
+// chunks of one XCD are contiguous: workgroup b runs on XCD b % 8, so the
+// blocks of one CTU, whose quadtree levels read the same region, meet in
+// one L2).  It validates them and sorts the blocks of sizes 8, 16, 32 and
+// 64 into slots of one size (64/L blocks, L lanes each), blocks whose
+// six-tap apron leaves the picture into slots of their own, and blocks of
+// other sizes into slots of 128 samples; the wavefronts take slots from a
+// counter in LDS, the slower kinds first.  A slot is one round trip to
+// memory: its aprons come into the wavefront's LDS and every half sample is
+// computed from them.
 __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__ table, int npics,
                                                       const tcme_pred *__restrict__ preds, int n,
                                                       uint8_t *__restrict__ out, int64_t out_bytes,
                                                       int32_t *__restrict__ status)
 {
-    const int lane = static_cast<int>(threadIdx.x & 63);
-    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    // classes: 0-3 interior 64/32/16/8, 4-7 edge 64/32/16/8, 8 other sizes
+    __shared__ McInfo info[kMcChunk];
+    __shared__ uint8_t lists[9][kMcChunk];
+    __shared__ int counts[9];
+    __shared__ int next;
+    __shared__ int oslot[kMcChunk + 1];
+    __shared__ u32x4l stage[4 * kMcWaveLds / 16 + 1];
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     constexpr int kXcd = 8;
     const int g = static_cast<int>(gridDim.x), bid = static_cast<int>(blockIdx.x);
-    const int q = g / kXcd, rr = g % kXcd, x = bid % kXcd;
-    const int chunk = x * q + (x < rr ? x : rr) + bid / kXcd;   // XCD x's chunks are contiguous
-    const int64_t c0 = static_cast<int64_t>(n) * chunk / g;
-    const int64_t c1 = static_cast<int64_t>(n) * (chunk + 1) / g;
-    PicDev R{};
-    int cached = -1;
-    for (int64_t w0 = c0; w0 < c1; w0 += 64) {
-        const int cnt = c1 - w0 < 64 ? static_cast<int>(c1 - w0) : 64;
-        tcme_pred p{};
-        bool ok = false;
-        uint64_t an = 0;
-        int apitch = 0, lg = 0;
-        if (lane < cnt) {
-            p = preds[w0 + lane];
-            lg = p.blk == 8 ? 3 : p.blk == 16 ? 4 : 0;
-            if (lg && p.pic >= 0 && p.pic < npics) {
-                const PicDev Rk = table[p.pic];
-                const int fx = p.mv_x >> 2, fy = p.mv_y >> 2;
-                ok = Rk.y && Rk.an && span_fits(p.out_off, p.blk, p.blk, p.out_stride, out_bytes) &&
-                     fx >= 2 && fy >= 2 && fx + p.blk + 3 <= Rk.w && fy + p.blk + 3 <= Rk.h;
-                an = reinterpret_cast<uint64_t>(Rk.an);
-                apitch = Rk.apitch;
+    const int q = g / kXcd, rr = g % kXcd, xc = bid % kXcd;
+    const int chunk = xc * q + (xc < rr ? xc : rr) + bid / kXcd;   // XCD xc's chunks are contiguous
+    const int64_t c0 = static_cast<int64_t>(chunk) * kMcChunk;
+    const int cnt = static_cast<int>(min(static_cast<int64_t>(kMcChunk), n - c0));
+    if (tid < 9) counts[tid] = 0;
+    if (tid == 0) next = 0;
+    __syncthreads();
+    if (tid < cnt) {
+        const tcme_pred p = preds[c0 + tid];
+        PicDev R{};
+        int st = 0;
+        if (p.pic < 0 || p.pic >= npics) {
+            st = TCME_E_PIC;
+        } else {
+            R = table[p.pic];
+            if (!R.y || !R.an) st = TCME_E_PIC;
+            else if (p.blk < 1 || p.blk > kMaxBlk) st = TCME_E_BLK;
+            else if (!span_fits(p.out_off, p.blk, p.blk, p.out_stride, out_bytes)) st = TCME_E_ARG;
+        }
+        if (status) status[c0 + tid] = st;
+        if (st == 0) {
+            const int fx = p.mv_x >> 2, fy = p.mv_y >> 2, nb = p.blk;
+            const bool inner = fx >= 2 && fy >= 2 && fx + nb + 3 <= R.w && fy + nb + 3 <= R.h;
+            const int sz = nb == 64 ? 0 : nb == 32 ? 1 : nb == 16 ? 2 : nb == 8 ? 3 : -1;
+            const int cls = sz < 0 ? 8 : inner ? sz : 4 + sz;
+            McInfo m;
+            m.src = reinterpret_cast<uint64_t>(
+                cls < 4 ? R.y + static_cast<size_t>(fy - 2) * R.pitch + ((fx - 2) & ~3) : R.y);
+            m.dst = reinterpret_cast<uint64_t>(out + p.out_off);
+            m.pitch = R.pitch;
+            m.frac = (p.mv_x & 3) | ((p.mv_y & 3) << 2) | (((fx - 2) & 3) << 4);
+            m.ostride = p.out_stride;
+            m.pic = p.pic;
+            m.fx = fx;
+            m.fy = fy;
+            m.w = R.w;
+            m.h = R.h;
+            info[tid] = m;
+            lists[cls][atomicAdd(&counts[cls], 1)] = static_cast<uint8_t>(tid);
+        }
+    }
+    __syncthreads();
+    const int no = counts[8];
+    if (no && wave == 0) {
+        // other-size blocks' first slots (128 samples a slot): a prefix sum
+        // over the list, 64 entries at a time
+        int base = 0;
+        for (int k0 = 0; k0 < no; k0 += 64) {
+            const int k = k0 + lane;
+            int v = 0;
+            if (k < no) {
+                const int nb = preds[c0 + lists[8][k]].blk;
+                v = (nb * nb + 127) >> 7;
             }
-        }
-        const int tasks = ok ? p.blk * p.blk / 8 : 0;
-        int incl = tasks;  // inclusive prefix sum over the lanes
+            int incl = v;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int t = shfl32(incl, lane >= d ? lane - d : lane);
-            if (lane >= d) incl += t;
-        }
-        const int excl = incl - tasks;
-        const int total = __builtin_amdgcn_readlane(incl, 63);
-        if (status && ok && wave == 0) status[w0 + lane] = 0;
-        for (int t0 = wave * 64; t0 < total; t0 += 256) {
-            // lanes past the last task take the last task's block and rows
-            // (so every address is one of an eligible block) and store nothing
-            const int t = t0 + lane, tt = t < total ? t : total - 1;
-            // the last descriptor whose tasks start at or before tt (excl is
-            // non-decreasing; descriptors without tasks share the next one's
-            // start and are passed over)
-            int k = 0;
-#pragma unroll
-            for (int sft = 32; sft; sft >>= 1) {
-                const int e = shfl32(excl, k + sft < 64 ? k + sft : 63);
-                if (k + sft < cnt && e <= tt) k += sft;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int t = __builtin_amdgcn_ds_bpermute((lane >= d ? lane - d : lane) << 2, incl);
+                if (lane >= d) incl += t;
             }
-            const int u = tt - shfl32(excl, k), kl = shfl32(lg, k);
-            const int mvx = shfl32(p.mv_x, k), mvy = shfl32(p.mv_y, k);
-            const int os = shfl32(p.out_stride, k), ap = shfl32(apitch, k);
-            const uint64_t a = shfl64(an, k), oo = shfl64(static_cast<uint64_t>(p.out_off), k);
-            if (t < total)
-                column8(reinterpret_cast<const uint64_t *>(a), ap, mvx, mvy, u & ((1 << kl) - 1),
-                        (u >> kl) * 8, out + oo, os);
+            if (k < no) oslot[k] = base + incl - v;
+            base += __builtin_amdgcn_readlane(incl, 63);
         }
-        // 32x32 and 64x64 blocks: every wavefront a quarter of the rows
-        uint64_t m = __ballot(lane < cnt && p.blk >= 32 && p.blk <= kMaxBlk);
-        while (m) {
-            const int k = __builtin_ctzll(m);
-            m &= m - 1;
-            const tcme_pred pk = preds[w0 + k];
-            const int band = (pk.blk + 3) / 4;  // luma_rows clips the last band
-            luma_rows(table, npics, pk, w0 + k, wave * band, (wave + 1) * band, out, out_bytes,
-                      status, R, cached, lane);
+        if (lane == 0) oslot[no] = base;
+    }
+    __syncthreads();
+    constexpr int G32 = McShape<32>::G, G16 = McShape<16>::G, G8 = McShape<8>::G;
+    // slots per class, in the order they are taken: edge 64 (4 bands a
+    // block), 32, 16, 8, other sizes, interior 64, 32, 16, 8
+    int ns[9];
+    ns[0] = 4 * counts[4];
+    ns[1] = (counts[5] + G32 - 1) / G32;
+    ns[2] = (counts[6] + G16 - 1) / G16;
+    ns[3] = (counts[7] + G8 - 1) / G8;
+    ns[4] = no ? oslot[no] : 0;
+    ns[5] = 4 * counts[0];
+    ns[6] = (counts[1] + G32 - 1) / G32;
+    ns[7] = (counts[2] + G16 - 1) / G16;
+    ns[8] = (counts[3] + G8 - 1) / G8;
+    int slots = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) slots += ns[k];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(stage) + wave * kMcWaveLds;
+    for (;;) {
+        int s = __builtin_amdgcn_readfirstlane(atomicAdd(&next, 1)) >> 6;
+        if (s >= slots) break;
+        // the lane index passes through an empty asm each round, so the
+        // compiler cannot hoist the slot kinds' lane-derived offsets out of
+        // the loop and hold them all live
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        const int n32e = counts[5], n16e = counts[6], n8e = counts[7];
+        if (s < ns[0]) { mc_slot<64, true>(info, lists[4] + (s >> 2), 1, s & 3, lds, ln); continue; }
+        s -= ns[0];
+        if (s < ns[1]) {
+            mc_slot<32, true>(info, lists[5] + G32 * s, min(G32, n32e - G32 * s), 0, lds, ln);
+            continue;
         }
-        // the rest (edges, other sizes, rejected descriptors), one
-        // descriptor per wavefront in turn
-        m = __ballot(lane < cnt && !ok && (p.blk < 32 || p.blk > kMaxBlk));
-        for (int j = 0; m; ++j) {
-            const int k = __builtin_ctzll(m);
-            m &= m - 1;
-            if ((j & 3) == wave)
-                luma_rows(table, npics, preds[w0 + k], w0 + k, 0, kMaxBlk, out, out_bytes, status,
-                          R, cached, lane);
+        s -= ns[1];
+        if (s < ns[2]) {
+            mc_slot<16, true>(info, lists[6] + G16 * s, min(G16, n16e - G16 * s), 0, lds, ln);
+            continue;
         }
+        s -= ns[2];
+        if (s < ns[3]) {
+            mc_slot<8, true>(info, lists[7] + G8 * s, min(G8, n8e - G8 * s), 0, lds, ln);
+            continue;
+        }
+        s -= ns[3];
+        if (s < ns[4]) {
+            // 128 samples of a block of another size, two per lane
+            int k = 0;  // the last block whose first slot is <= s
+            for (int sft = 32; sft; sft >>= 1)
+                if (k + sft < no && oslot[k + sft] <= s) k += sft;
+            const McInfo m = info[lists[8][k]];
+            const PicDev R = table[m.pic];
+            const int dx = m.frac & 3, dy = (m.frac >> 2) & 3;
+            const int nb = preds[c0 + lists[8][k]].blk, tot = nb * nb;
+            const gout8 o = (gout8)m.dst;
+            const int u0 = (s - oslot[k]) * 128;
+            const int ua = u0 + ln, ub = u0 + 64 + ln;
+            const int ca = ua % nb, ra = min(ua / nb, nb - 1), cb = ub % nb, rb = min(ub / nb, nb - 1);
+            const int va = luma_px_flat(R, m.fx + ca, m.fy + ra, dx, dy);
+            const int vb = luma_px_flat(R, m.fx + cb, m.fy + rb, dx, dy);
+            if (ua < tot) o[static_cast<int64_t>(ra) * m.ostride + ca] = static_cast<uint8_t>(va);
+            if (ub < tot) o[static_cast<int64_t>(rb) * m.ostride + cb] = static_cast<uint8_t>(vb);
+            continue;
+        }
+        s -= ns[4];
+        const int n32 = counts[1], n16 = counts[2], n8 = counts[3];
+        if (s < ns[5]) { mc_slot<64, false>(info, lists[0] + (s >> 2), 1, s & 3, lds, ln); continue; }
+        s -= ns[5];
+        if (s < ns[6]) {
+            mc_slot<32, false>(info, lists[1] + G32 * s, min(G32, n32 - G32 * s), 0, lds, ln);
+            continue;
+        }
+        s -= ns[6];
+        if (s < ns[7]) {
+            mc_slot<16, false>(info, lists[2] + G16 * s, min(G16, n16 - G16 * s), 0, lds, ln);
+            continue;
+        }
+        s -= ns[7];
+        mc_slot<8, false>(info, lists[3] + G8 * s, min(G8, n8 - G8 * s), 0, lds, ln);
     }
 }
 
@@ -556,8 +695,8 @@ hipError_t launch_predict(bool chroma, const PicDev *table, int npics, const tcm
         hipLaunchKernelGGL(mc_chroma_kernel, grid, dim3(256), 0, s, table, npics, preds, n, out,
                            out_bytes, status);
     else
-        hipLaunchKernelGGL(mc_luma_kernel, grid, dim3(256), 0, s, table, npics, preds, n, out,
-                           out_bytes, status);
+        hipLaunchKernelGGL(mc_luma_kernel, dim3((n + kMcChunk - 1) / kMcChunk), dim3(256), 0, s,
+                           table, npics, preds, n, out, out_bytes, status);
     return hipGetLastError();
 }
 

@@ -115,6 +115,37 @@ def test_gpu_luma_mc_interior_blocks(engine, orc):
     engine.free(rp)
 
 
+def test_gpu_luma_mc_far_outside_and_straddling(engine, orc):
+    """Blocks of 8 .. 64 whose six-tap window leaves the picture: MVs up to
+    2,000 samples past every edge and corner (every sample clamps to the
+    border, far outside the apron the kernel stages), and blocks straddling
+    each edge at every phase (six-tap, bilinear and nearest samples in one
+    block)."""
+    h, w = 90, 150
+    print(f"luma mc far outside and straddling {w}x{h}", flush=True)
+    rng = np.random.default_rng(41)
+    ref, _ = moving_pair(h, w, 3)
+    rp = engine.pic(ref)
+    sizes, mvs = [], []
+    for n in (8, 16, 32, 64):
+        for dxq in (-2000, -300, -n - 7, -n - 2, -3, -1, 0, 1, 2, w - n - 2, w - n + 1, w + 4, w + 900):
+            for dyq in (-1500, -n - 3, -2, 0, 3, h - n - 3, h - n + 2, h + 700):
+                ph = int(rng.integers(0, 16))
+                sizes.append(n)
+                mvs.append((dxq * 4 + ph % 4, dyq * 4 + ph // 4))
+    sizes, mvs = np.array(sizes), np.array(mvs)
+    preds = _preds(rp, mvs, sizes)
+    out, st = engine.predict(preds)
+    assert (st == 0).all()
+    for k in range(len(preds)):
+        n = int(sizes[k])
+        o = int(preds["out_off"][k])
+        got = out[o:o + n * n].reshape(n, n)
+        want = orc.inter_predict(ref, int(mvs[k, 0]), int(mvs[k, 1]), n)
+        assert np.array_equal(got, want), (k, mvs[k], n)
+    engine.free(rp)
+
+
 def test_gpu_luma_mc_every_position_and_phase(engine, orc):
     """1x1 predictions at every position of a 70x37 frame (two anchor tiles
     across, three down) at each of the 16 quarter-sample phases: an interior
