@@ -138,10 +138,10 @@ struct McShape {
 
 // Per-descriptor facts a slot needs, in LDS (48 bytes).
 struct McInfo {
-    uint64_t src;     // interior: the apron's first byte (aligned down to 4); else the plane's
+    uint64_t src;     // the plane's first byte
     uint64_t dst;     // the block's first output byte
     int32_t pitch;    // plane pitch
-    int32_t frac;     // dx | dy << 2 | ((fx - 2) & 3) << 4
+    int32_t frac;     // dx | dy << 2 | ((fx - 2) & 3) << 4 | edge << 6
     int32_t ostride;  // output row stride
     int32_t pic;      // picture id
     int32_t fx, fy;   // integer part of the MV
@@ -181,6 +181,14 @@ __device__ __forceinline__ int luma_px_flat(const PicDev &R, int x, int y, int d
     return pn;
 }
 
+// What a block asks of the task code: KIND bit 0, dx even (X0 = X1: one
+// lattice column per row); bit 1, dy = 0 (no vertical half samples, apron
+// rows 2..9 only).  Kind 0 is the general case and serves any mix.
+__device__ __forceinline__ int mc_kind(int dx, int dy)
+{
+    return ((dx & 1) == 0 ? 1 : 0) | (dy == 0 ? 2 : 0);
+}
+
 // Rows r0 .. r0+7 of columns c, c+1 of a staged block (apron at lds, row
 // pitch P, apron column 0 at byte a of each row) at fraction (dx, dy), the
 // two columns in the 16-bit halves of res[r].  Each of the task's 13 apron
@@ -191,11 +199,13 @@ __device__ __forceinline__ int luma_px_flat(const PicDev &R, int x, int y, int d
 // columns (vv of a sample column, cc of an hh column), so with hE(y) =
 // avg(e0, e1) on row y and hO(y) = avg(v6(e0), v6(e1)) the value is
 // clip(avg(top, bot)) with top = hE(y) or hO(y) and bot = hE(y), hO(y) or
-// hE(y+1), by dy.
-template <int P>
-__device__ __forceinline__ void mc_task(const uint8_t *lds, int a, int dx, int dy, int c, int r0,
-                                        uint32_t (&res)[8])
+// hE(y+1), by dy.  KIND (mc_kind) drops what the block does not need.
+template <int KIND>
+__device__ __forceinline__ void mc_task(const uint8_t *lds, int P, int a, int dx, int dy, int c,
+                                        int r0, uint32_t (&res)[8])
 {
+    constexpr bool ONE = (KIND & 1) != 0, FLAT = (KIND & 2) != 0;
+    constexpr int RB = FLAT ? 2 : 0, RE = FLAT ? 10 : 13;  // apron rows used
     const int off = a + c, sh = off & 3;
     const uint8_t *b = lds + r0 * P + (off & ~3);
     constexpr uint32_t kM = 0x80808080u;
@@ -204,68 +214,95 @@ __device__ __forceinline__ void mc_task(const uint8_t *lds, int a, int dx, int d
     const bool xa = X0 != 0, xb0 = X1 == 0, xb2 = X1 == 2;
     uint32_t E0[13], E1[13];
 #pragma unroll
-    for (int i = 0; i < 13; ++i) {
+    for (int i = RB; i < RE; ++i) {
         const uint32_t *w = reinterpret_cast<const uint32_t *>(b + i * P);
         const uint32_t d0 = w[0], d1 = w[1], d2 = w[2];
         const uint32_t w0 = __builtin_amdgcn_alignbyte(d1, d0, sh);
         const uint32_t w1 = __builtin_amdgcn_alignbyte(d2, d1, sh);
         const uint32_t u0 = w0 ^ kM, u1 = w1 ^ kM;
         const int h0 = __builtin_amdgcn_sdot4(static_cast<int>(u0), K0,
-                                              __builtin_amdgcn_sdot4(static_cast<int>(u1), K1, 4112, false), false);
-        const int h1 = __builtin_amdgcn_sdot4(static_cast<int>(__builtin_amdgcn_alignbyte(u1, u0, 1)), K0,
-                                              __builtin_amdgcn_sdot4(static_cast<int>(u1 >> 8), K1, 4112, false),
+                                              __builtin_amdgcn_sdot4(static_cast<int>(u1), K1,
+                                                                     4112, false),
                                               false);
+        const int h1 = __builtin_amdgcn_sdot4(
+            static_cast<int>(__builtin_amdgcn_alignbyte(u1, u0, 1)), K0,
+            __builtin_amdgcn_sdot4(static_cast<int>(u1 >> 8), K1, 4112, false), false);
         // (column c, column c+1) in the two 16-bit halves
-        const uint32_t sp = __builtin_amdgcn_perm(w1, w0, 0x0c030c02u);   // s(x), s(x+1)
-        const uint32_t sq = __builtin_amdgcn_perm(w1, w0, 0x0c040c03u);   // s(x+1), s(x+2)
+        const uint32_t sp = __builtin_amdgcn_perm(w1, w0, 0x0c030c02u);  // s(x), s(x+1)
         const uint32_t hp = as_u32(as_pk16(__builtin_amdgcn_perm(static_cast<uint32_t>(h1),
-                                                                 static_cast<uint32_t>(h0), 0x05040100u)) >> 5);
+                                                                 static_cast<uint32_t>(h0),
+                                                                 0x05040100u)) >> 5);
         E0[i] = xa ? hp : sp;
-        E1[i] = xb0 ? sp : (xb2 ? sq : hp);
+        if constexpr (!ONE) {
+            const uint32_t sq = __builtin_amdgcn_perm(w1, w0, 0x0c040c03u);  // s(x+1), s(x+2)
+            E1[i] = xb0 ? sp : (xb2 ? sq : hp);
+        }
     }
-    const bool yt = dy >= 2, y1e = dy == 0, y1f = dy == 3;
     const pk16 one = {1, 1}, lo = {0, 0}, hi = {255, 255};
-    pk16 hE = (as_pk16(E0[2]) + as_pk16(E1[2]) + one) >> 1;
+    if constexpr (FLAT) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const pk16 o0 = v6(as_pk16(E0[r]), as_pk16(E0[r + 1]), as_pk16(E0[r + 2]), as_pk16(E0[r + 3]),
-                           as_pk16(E0[r + 4]), as_pk16(E0[r + 5]));
-        const pk16 o1 = v6(as_pk16(E1[r]), as_pk16(E1[r + 1]), as_pk16(E1[r + 2]), as_pk16(E1[r + 3]),
-                           as_pk16(E1[r + 4]), as_pk16(E1[r + 5]));
-        const pk16 hO = (o0 + o1 + one) >> 1;
-        const pk16 hF = (as_pk16(E0[r + 3]) + as_pk16(E1[r + 3]) + one) >> 1;
-        const pk16 top = yt ? hO : hE;
-        const pk16 bot = y1e ? hE : (y1f ? hF : hO);
-        const pk16 v = __builtin_elementwise_min(__builtin_elementwise_max((top + bot + one) >> 1, lo), hi);
-        res[r] = as_u32(v);
-        hE = hF;
+        for (int r = 0; r < 8; ++r) {
+            pk16 v = as_pk16(E0[r + 2]);
+            if constexpr (!ONE) v = (v + as_pk16(E1[r + 2]) + one) >> 1;
+            res[r] = as_u32(__builtin_elementwise_min(__builtin_elementwise_max(v, lo), hi));
+        }
+    } else {
+        const bool yt = dy >= 2, y1e = dy == 0, y1f = dy == 3;
+        pk16 hE = as_pk16(E0[2]);
+        if constexpr (!ONE) hE = (hE + as_pk16(E1[2]) + one) >> 1;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            pk16 hO = v6(as_pk16(E0[r]), as_pk16(E0[r + 1]), as_pk16(E0[r + 2]),
+                         as_pk16(E0[r + 3]), as_pk16(E0[r + 4]), as_pk16(E0[r + 5]));
+            pk16 hF = as_pk16(E0[r + 3]);
+            if constexpr (!ONE) {
+                const pk16 o1 = v6(as_pk16(E1[r]), as_pk16(E1[r + 1]), as_pk16(E1[r + 2]),
+                                   as_pk16(E1[r + 3]), as_pk16(E1[r + 4]), as_pk16(E1[r + 5]));
+                hO = (hO + o1 + one) >> 1;
+                hF = (hF + as_pk16(E1[r + 3]) + one) >> 1;
+            }
+            const pk16 top = yt ? hO : hE;
+            const pk16 bot = y1e ? hE : (y1f ? hF : hO);
+            const pk16 v = (top + bot + one) >> 1;
+            res[r] = as_u32(__builtin_elementwise_min(__builtin_elementwise_max(v, lo), hi));
+            hE = hF;
+        }
     }
 }
 
-// One slot: up to G blocks of size N (descriptor numbers idx[0..cnt) of the
-// workgroup's chunk; for N = 64 the band of 16 rows `band`), L lanes each,
-// one task per lane.  The lanes stage their block's apron in the
-// wavefront's own LDS region, then compute their task from it.
-//   !EDGE: the apron lies inside the plane: 16-byte loads.
-//   EDGE:  it may not: dword loads with the row clamped into the picture and
-//          the column into the pitch (a clamped dword lies wholly outside the
-//          picture, as the columns are dword-aligned, so every in-picture
-//          byte of the copy is exact); a sample whose 6x6 window is in the
-//          picture keeps the six-tap value, any other one takes the
-//          ladder's bilinear or nearest value from the copy (from the plane
-//          when the nearest sample lies outside the copy).
-template <int N, bool EDGE>
-__device__ __forceinline__ void mc_slot(const McInfo *info, const uint8_t *idx, int cnt, int band,
-                                        uint8_t *lds, int lane)
+// A lane's task of a slot: 2 columns x 8 rows of one block, its apron staged
+// at lds (row pitch P).
+struct McTask {
+    const uint8_t *lds;
+    int P, rows;   // LDS row pitch, staged rows
+    int rb, c, r0; // band's first block row, the task's column pair and rows
+    bool act;      // the lane's group holds a block
+};
+
+// Stages one slot: up to G blocks of size N (descriptor numbers idx[0..cnt)
+// of the workgroup's chunk; for N = 64 the band of 16 rows `band`), L lanes
+// each.  The lanes stage their block's apron in the wavefront's own LDS
+// region and return their task; `edge` tells whether a block of the slot
+// has an apron reaching past the picture.
+//   no such block: 16-byte loads;
+//   else: dword loads with the row clamped into the picture and the column
+//         into the pitch (a clamped dword lies wholly outside the picture,
+//         as the columns are dword-aligned, so every in-picture byte of the
+//         copy is exact).
+template <int N>
+__device__ __forceinline__ McTask mc_stage(const McInfo *info, const uint8_t *idx, int cnt,
+                                           int band, uint8_t *lds, int lane, McInfo &m, bool &edge)
 {
     using S = McShape<N>;
     const int g = lane / S::L, li = lane % S::L;
     const bool act = g < cnt;
-    const McInfo m = info[idx[act ? g : 0]];
+    m = info[idx[act ? g : 0]];
+    edge = __ballot(act && (m.frac & 64)) != 0;
     uint8_t *blds = lds + g * S::ROWS * S::P;
     const int rb = band * S::BR;  // the band's first block row
-    if constexpr (!EDGE) {
-        const gptr8 src = (gptr8)m.src + static_cast<size_t>(rb) * m.pitch;
+    const int xs = (m.fx - 2) & ~3;
+    if (!edge) {
+        const gptr8 src = (gptr8)m.src + static_cast<size_t>(m.fy - 2 + rb) * m.pitch + xs;
         // every lane loads (segments past the apron re-read its last one,
         // the groups without a block read the slot's first block), so the
         // loads are issued together
@@ -286,7 +323,6 @@ __device__ __forceinline__ void mc_slot(const McInfo *info, const uint8_t *idx, 
     } else {
         constexpr int D4 = S::P / 4, DS = S::ROWS * D4, DPL = (DS + S::L - 1) / S::L;
         const gptr8 src = (gptr8)m.src;
-        const int xs = (m.fx - 2) & ~3;
         uint32_t v[DPL];
 #pragma unroll
         for (int i = 0; i < DPL; ++i) {
@@ -304,46 +340,88 @@ __device__ __forceinline__ void mc_slot(const McInfo *info, const uint8_t *idx, 
             *reinterpret_cast<uint32_t *>(blds + row * S::P + 4 * dc) = v[i];
         }
     }
+    McTask t;
+    t.lds = blds;
+    t.P = S::P;
+    t.rows = S::ROWS;
+    t.rb = rb;
+    t.c = 2 * (li % S::PAIRS);
+    t.r0 = 8 * (li / S::PAIRS);
+    t.act = act;
+    return t;
+}
+
+// Computes a staged slot's tasks and stores them (a row's two samples in
+// one 16-bit store when aligned).  The slot's blocks share one code path:
+// the specialised one of their kind when they all have the same (mc_kind),
+// else the general one.  `edge` (the slot's blocks may reach past the
+// picture): a sample whose 6x6 window is in the picture keeps the six-tap
+// value, any other one is stored again with the ladder's bilinear (2x2 in
+// the picture) or nearest value from the LDS copy (in-picture bytes are
+// exact there), or from the plane when the nearest sample lies outside the
+// copy; the same lane stores both, in order.
+__device__ __forceinline__ void mc_compute(const McTask &t, const McInfo &m, bool edge)
+{
     // the wavefront's LDS writes before its reads (one wavefront owns the region)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (act) {
+    if (t.act) {
         const int dx = m.frac & 3, dy = (m.frac >> 2) & 3, a = (m.frac >> 4) & 3;
-        const int c = 2 * (li % S::PAIRS), r0 = 8 * (li / S::PAIRS);
+        const int kind = mc_kind(dx, dy);
+        const int k0 = __builtin_amdgcn_readfirstlane(kind);
+        const int kk = __ballot(kind != k0) ? 0 : k0;
         uint32_t res[8];
-        mc_task<S::P>(blds, a, dx, dy, c, r0, res);
-        const gout8 oc = (gout8)m.dst + static_cast<int64_t>(rb + r0) * m.ostride + c;
-        if constexpr (!EDGE) {
+        switch (kk) {
+        case 1: mc_task<1>(t.lds, t.P, a, dx, dy, t.c, t.r0, res); break;
+        case 2: mc_task<2>(t.lds, t.P, a, dx, dy, t.c, t.r0, res); break;
+        case 3: mc_task<3>(t.lds, t.P, a, dx, dy, t.c, t.r0, res); break;
+        default: mc_task<0>(t.lds, t.P, a, dx, dy, t.c, t.r0, res); break;
+        }
+        const gout8 oc = (gout8)m.dst + static_cast<int64_t>(t.rb + t.r0) * m.ostride + t.c;
+        if (((m.dst + t.c) & 1) == 0 && (m.ostride & 1) == 0) {
+            // both columns of a row in one 16-bit store
+            typedef __attribute__((address_space(1))) uint16_t *gout16;
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                *(gout16)(oc + static_cast<int64_t>(r) * m.ostride) =
+                    static_cast<uint16_t>((res[r] & 0xffu) | ((res[r] >> 8) & 0xff00u));
+        } else {
 #pragma unroll
             for (int r = 0; r < 8; ++r) {
                 oc[static_cast<int64_t>(r) * m.ostride] = static_cast<uint8_t>(res[r]);
                 oc[static_cast<int64_t>(r) * m.ostride + 1] = static_cast<uint8_t>(res[r] >> 16);
             }
-        } else {
-            // the six-tap value holds where the sample's 6x6 window is in the
-            // picture; elsewhere the ladder's bilinear (2x2 in the picture)
-            // or nearest sample, read from the LDS copy (in-picture bytes
-            // are exact there), or from the plane when the nearest sample
-            // lies outside the copy
-            const int xs = (m.fx - 2) & ~3, ys = m.fy - 2 + rb;
-#pragma unroll 1
-            for (int k = 0; k < 16; ++k) {
+        }
+        if (edge && (m.frac & 64)) {
+            // samples whose 6x6 window leaves the picture (bit 2r + column),
+            // stored again with the ladder's value
+            const int x = m.fx + t.c, y = m.fy + t.rb + t.r0;
+            uint32_t colok = 0, rowok = 0;
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc)
+                if (x + cc >= 2 && x + cc + 3 < m.w) colok |= 0x5555u << cc;
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+                if (y + r >= 2 && y + r + 3 < m.h) rowok |= 3u << (2 * r);
+            uint32_t fix = ~(colok & rowok) & 0xffffu;
+            const int xs = (m.fx - 2) & ~3, ys = m.fy - 2 + t.rb;
+            while (fix) {
+                const int k = __builtin_ctz(fix);
+                fix &= fix - 1;
                 const int r = k >> 1, cc = k & 1;
-                const int x = m.fx + c + cc, y = m.fy + rb + r0 + r;
-                int v = static_cast<int>((res[r] >> (16 * cc)) & 0xff);
-                if (!(x >= 2 && x + 3 < m.w && y >= 2 && y + 3 < m.h)) {
-                    if (x >= 0 && y >= 0 && x + 1 < m.w && y + 1 < m.h) {
-                        const uint8_t *p = blds + (y - ys) * S::P + (x - xs);
-                        v = lerp4(lerp4(p[0], p[1], dx), lerp4(p[S::P], p[S::P + 1], dx), dy);
-                    } else {
-                        const int nx = min(max(x + (dx >= 2 ? 1 : 0), 0), m.w - 1);
-                        const int ny = min(max(y + (dy >= 2 ? 1 : 0), 0), m.h - 1);
-                        const int X = nx - xs, Y = ny - ys;
-                        v = X >= 0 && X < S::P && Y >= 0 && Y < S::ROWS
-                                ? blds[Y * S::P + X]
-                                : ((gptr8)m.src)[static_cast<size_t>(ny) * m.pitch + nx];
-                    }
+                const int xk = x + cc, yk = y + r;
+                int v;
+                if (xk >= 0 && yk >= 0 && xk + 1 < m.w && yk + 1 < m.h) {
+                    const uint8_t *q = t.lds + (yk - ys) * t.P + (xk - xs);
+                    v = lerp4(lerp4(q[0], q[1], dx), lerp4(q[t.P], q[t.P + 1], dx), dy);
+                } else {
+                    const int nx = min(max(xk + (dx >= 2 ? 1 : 0), 0), m.w - 1);
+                    const int ny = min(max(yk + (dy >= 2 ? 1 : 0), 0), m.h - 1);
+                    const int X = nx - xs, Y = ny - ys;
+                    v = X >= 0 && X < t.P && Y >= 0 && Y < t.rows
+                            ? t.lds[Y * t.P + X]
+                            : ((gptr8)m.src)[static_cast<size_t>(ny) * m.pitch + nx];
                 }
                 oc[static_cast<int64_t>(r) * m.ostride + cc] = static_cast<uint8_t>(v);
             }
@@ -358,21 +436,23 @@ __device__ __forceinline__ void mc_slot(const McInfo *info, const uint8_t *idx, 
 // chunks of one XCD are contiguous: workgroup b runs on XCD b % 8, so the
 // blocks of one CTU, whose quadtree levels read the same region, meet in
 // one L2).  It validates them and sorts the blocks of sizes 8, 16, 32 and
-// 64 into slots of one size (64/L blocks, L lanes each), blocks whose
-// six-tap apron leaves the picture into slots of their own, and blocks of
-// other sizes into slots of 128 samples; the wavefronts take slots from a
-// counter in LDS, the slower kinds first.  A slot is one round trip to
-// memory: its aprons come into the wavefront's LDS and every half sample is
-// computed from them.
+// 64 into slots of one size (64/L blocks, L lanes each), each size's list
+// ordered by whether the block's apron reaches past the picture and by
+// kind (mc_kind), so that a slot's blocks mostly share their staging and
+// code path; blocks of other sizes go into slots of 128 samples.  The
+// wavefronts take slots from a counter in LDS, the slowest first.  A slot
+// is one round trip to memory: its aprons come into the wavefront's LDS and
+// every half sample is computed from them.
 __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__ table, int npics,
                                                       const tcme_pred *__restrict__ preds, int n,
                                                       uint8_t *__restrict__ out, int64_t out_bytes,
                                                       int32_t *__restrict__ status)
 {
-    // classes: 0-3 interior 64/32/16/8, 4-7 edge 64/32/16/8, 8 other sizes
+    // classes: 0-3 sizes 64/32/16/8, 4 other sizes; keys: edge << 2 | kind
     __shared__ McInfo info[kMcChunk];
-    __shared__ uint8_t lists[9][kMcChunk];
-    __shared__ int counts[9];
+    __shared__ uint8_t lists[5][kMcChunk];
+    __shared__ int counts[5];
+    __shared__ int kcount[5][8];
     __shared__ int next;
     __shared__ int oslot[kMcChunk + 1];
     __shared__ u32x4l stage[4 * kMcWaveLds / 16 + 1];
@@ -384,9 +464,10 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
     const int chunk = xc * q + (xc < rr ? xc : rr) + bid / kXcd;   // XCD xc's chunks are contiguous
     const int64_t c0 = static_cast<int64_t>(chunk) * kMcChunk;
     const int cnt = static_cast<int>(min(static_cast<int64_t>(kMcChunk), n - c0));
-    if (tid < 9) counts[tid] = 0;
+    if (tid < 40) kcount[tid >> 3][tid & 7] = 0;
     if (tid == 0) next = 0;
     __syncthreads();
+    int cl = -1, key = 0, rank = 0;
     if (tid < cnt) {
         const tcme_pred p = preds[c0 + tid];
         PicDev R{};
@@ -403,14 +484,13 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
         if (st == 0) {
             const int fx = p.mv_x >> 2, fy = p.mv_y >> 2, nb = p.blk;
             const bool inner = fx >= 2 && fy >= 2 && fx + nb + 3 <= R.w && fy + nb + 3 <= R.h;
-            const int sz = nb == 64 ? 0 : nb == 32 ? 1 : nb == 16 ? 2 : nb == 8 ? 3 : -1;
-            const int cls = sz < 0 ? 8 : inner ? sz : 4 + sz;
+            const int sz = nb == 64 ? 0 : nb == 32 ? 1 : nb == 16 ? 2 : nb == 8 ? 3 : 4;
             McInfo m;
-            m.src = reinterpret_cast<uint64_t>(
-                cls < 4 ? R.y + static_cast<size_t>(fy - 2) * R.pitch + ((fx - 2) & ~3) : R.y);
+            m.src = reinterpret_cast<uint64_t>(R.y);
             m.dst = reinterpret_cast<uint64_t>(out + p.out_off);
             m.pitch = R.pitch;
-            m.frac = (p.mv_x & 3) | ((p.mv_y & 3) << 2) | (((fx - 2) & 3) << 4);
+            m.frac = (p.mv_x & 3) | ((p.mv_y & 3) << 2) | (((fx - 2) & 3) << 4) |
+                     (inner ? 0 : 64);
             m.ostride = p.out_stride;
             m.pic = p.pic;
             m.fx = fx;
@@ -418,11 +498,24 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
             m.w = R.w;
             m.h = R.h;
             info[tid] = m;
-            lists[cls][atomicAdd(&counts[cls], 1)] = static_cast<uint8_t>(tid);
+            cl = sz;
+            key = (inner ? 0 : 4) | mc_kind(p.mv_x & 3, p.mv_y & 3);
+            rank = atomicAdd(&kcount[sz][key], 1);
         }
     }
     __syncthreads();
-    const int no = counts[8];
+    if (cl >= 0) {
+        int pos = rank;
+        for (int k = 0; k < key; ++k) pos += kcount[cl][k];
+        lists[cl][pos] = static_cast<uint8_t>(tid);
+    }
+    if (tid < 5) {
+        int c = 0;
+        for (int k = 0; k < 8; ++k) c += kcount[tid][k];
+        counts[tid] = c;
+    }
+    __syncthreads();
+    const int no = counts[4];
     if (no && wave == 0) {
         // other-size blocks' first slots (128 samples a slot): a prefix sum
         // over the list, 64 entries at a time
@@ -431,7 +524,7 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
             const int k = k0 + lane;
             int v = 0;
             if (k < no) {
-                const int nb = preds[c0 + lists[8][k]].blk;
+                const int nb = preds[c0 + lists[4][k]].blk;
                 v = (nb * nb + 127) >> 7;
             }
             int incl = v;
@@ -447,82 +540,56 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
     }
     __syncthreads();
     constexpr int G32 = McShape<32>::G, G16 = McShape<16>::G, G8 = McShape<8>::G;
-    // slots per class, in the order they are taken: edge 64 (4 bands a
-    // block), 32, 16, 8, other sizes, interior 64, 32, 16, 8
-    int ns[9];
-    ns[0] = 4 * counts[4];
-    ns[1] = (counts[5] + G32 - 1) / G32;
-    ns[2] = (counts[6] + G16 - 1) / G16;
-    ns[3] = (counts[7] + G8 - 1) / G8;
-    ns[4] = no ? oslot[no] : 0;
-    ns[5] = 4 * counts[0];
-    ns[6] = (counts[1] + G32 - 1) / G32;
-    ns[7] = (counts[2] + G16 - 1) / G16;
-    ns[8] = (counts[3] + G8 - 1) / G8;
-    int slots = 0;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) slots += ns[k];
+    const int n64 = counts[0], n32 = counts[1], n16 = counts[2], n8 = counts[3];
+    // slots in the order they are taken: other sizes, 64 (4 bands a block),
+    // 32, 16, 8
+    const int so = no ? oslot[no] : 0, s64 = 4 * n64, s32 = (n32 + G32 - 1) / G32,
+              s16 = (n16 + G16 - 1) / G16, s8 = (n8 + G8 - 1) / G8;
+    const int slots = so + s64 + s32 + s16 + s8;
     uint8_t *lds = reinterpret_cast<uint8_t *>(stage) + wave * kMcWaveLds;
     for (;;) {
         int s = __builtin_amdgcn_readfirstlane(atomicAdd(&next, 1)) >> 6;
         if (s >= slots) break;
         // the lane index passes through an empty asm each round, so the
-        // compiler cannot hoist the slot kinds' lane-derived offsets out of
+        // compiler cannot hoist the slot shapes' lane-derived offsets out of
         // the loop and hold them all live
         int ln = lane;
         asm volatile("" : "+v"(ln));
-        const int n32e = counts[5], n16e = counts[6], n8e = counts[7];
-        if (s < ns[0]) { mc_slot<64, true>(info, lists[4] + (s >> 2), 1, s & 3, lds, ln); continue; }
-        s -= ns[0];
-        if (s < ns[1]) {
-            mc_slot<32, true>(info, lists[5] + G32 * s, min(G32, n32e - G32 * s), 0, lds, ln);
-            continue;
-        }
-        s -= ns[1];
-        if (s < ns[2]) {
-            mc_slot<16, true>(info, lists[6] + G16 * s, min(G16, n16e - G16 * s), 0, lds, ln);
-            continue;
-        }
-        s -= ns[2];
-        if (s < ns[3]) {
-            mc_slot<8, true>(info, lists[7] + G8 * s, min(G8, n8e - G8 * s), 0, lds, ln);
-            continue;
-        }
-        s -= ns[3];
-        if (s < ns[4]) {
+        if (s < so) {
             // 128 samples of a block of another size, two per lane
             int k = 0;  // the last block whose first slot is <= s
             for (int sft = 32; sft; sft >>= 1)
                 if (k + sft < no && oslot[k + sft] <= s) k += sft;
-            const McInfo m = info[lists[8][k]];
+            const McInfo m = info[lists[4][k]];
             const PicDev R = table[m.pic];
             const int dx = m.frac & 3, dy = (m.frac >> 2) & 3;
-            const int nb = preds[c0 + lists[8][k]].blk, tot = nb * nb;
+            const int nb = preds[c0 + lists[4][k]].blk, tot = nb * nb;
             const gout8 o = (gout8)m.dst;
             const int u0 = (s - oslot[k]) * 128;
             const int ua = u0 + ln, ub = u0 + 64 + ln;
-            const int ca = ua % nb, ra = min(ua / nb, nb - 1), cb = ub % nb, rb = min(ub / nb, nb - 1);
+            const int ca = ua % nb, ra = min(ua / nb, nb - 1);
+            const int cb = ub % nb, rb = min(ub / nb, nb - 1);
             const int va = luma_px_flat(R, m.fx + ca, m.fy + ra, dx, dy);
             const int vb = luma_px_flat(R, m.fx + cb, m.fy + rb, dx, dy);
             if (ua < tot) o[static_cast<int64_t>(ra) * m.ostride + ca] = static_cast<uint8_t>(va);
             if (ub < tot) o[static_cast<int64_t>(rb) * m.ostride + cb] = static_cast<uint8_t>(vb);
             continue;
         }
-        s -= ns[4];
-        const int n32 = counts[1], n16 = counts[2], n8 = counts[3];
-        if (s < ns[5]) { mc_slot<64, false>(info, lists[0] + (s >> 2), 1, s & 3, lds, ln); continue; }
-        s -= ns[5];
-        if (s < ns[6]) {
-            mc_slot<32, false>(info, lists[1] + G32 * s, min(G32, n32 - G32 * s), 0, lds, ln);
-            continue;
+        s -= so;
+        McInfo m;
+        McTask t;
+        bool edge;
+        if (s < s64) {
+            t = mc_stage<64>(info, lists[0] + (s >> 2), 1, s & 3, lds, ln, m, edge);
+        } else if ((s -= s64) < s32) {
+            t = mc_stage<32>(info, lists[1] + G32 * s, min(G32, n32 - G32 * s), 0, lds, ln, m, edge);
+        } else if ((s -= s32) < s16) {
+            t = mc_stage<16>(info, lists[2] + G16 * s, min(G16, n16 - G16 * s), 0, lds, ln, m, edge);
+        } else {
+            s -= s16;
+            t = mc_stage<8>(info, lists[3] + G8 * s, min(G8, n8 - G8 * s), 0, lds, ln, m, edge);
         }
-        s -= ns[6];
-        if (s < ns[7]) {
-            mc_slot<16, false>(info, lists[2] + G16 * s, min(G16, n16 - G16 * s), 0, lds, ln);
-            continue;
-        }
-        s -= ns[7];
-        mc_slot<8, false>(info, lists[3] + G8 * s, min(G8, n8 - G8 * s), 0, lds, ln);
+        mc_compute(t, m, edge);
     }
 }
 
