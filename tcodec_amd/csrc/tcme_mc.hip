@@ -432,7 +432,7 @@ __device__ __forceinline__ void mc_compute(const McTask &t, const McInfo &m, boo
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Workgroup b (4 wavefronts) takes kMcChunk consecutive descriptors (the
+// Workgroup b (4 wavefronts) takes up to kMcChunk consecutive descriptors (the
 // chunks of one XCD are contiguous: workgroup b runs on XCD b % 8, so the
 // blocks of one CTU, whose quadtree levels read the same region, meet in
 // one L2).  It validates them and sorts the blocks of sizes 8, 16, 32 and
@@ -462,8 +462,10 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
     const int g = static_cast<int>(gridDim.x), bid = static_cast<int>(blockIdx.x);
     const int q = g / kXcd, rr = g % kXcd, xc = bid % kXcd;
     const int chunk = xc * q + (xc < rr ? xc : rr) + bid / kXcd;   // XCD xc's chunks are contiguous
-    const int64_t c0 = static_cast<int64_t>(chunk) * kMcChunk;
-    const int cnt = static_cast<int>(min(static_cast<int64_t>(kMcChunk), n - c0));
+    // an even split: at most kMcChunk descriptors a workgroup, and at least
+    // 8 workgroups a CU, so that a short list of large blocks still spreads
+    const int64_t c0 = static_cast<int64_t>(n) * chunk / g;
+    const int cnt = static_cast<int>(static_cast<int64_t>(n) * (chunk + 1) / g - c0);
     if (tid < 40) kcount[tid >> 3][tid & 7] = 0;
     if (tid == 0) next = 0;
     __syncthreads();
@@ -762,8 +764,9 @@ hipError_t launch_predict(bool chroma, const PicDev *table, int npics, const tcm
         hipLaunchKernelGGL(mc_chroma_kernel, grid, dim3(256), 0, s, table, npics, preds, n, out,
                            out_bytes, status);
     else
-        hipLaunchKernelGGL(mc_luma_kernel, dim3((n + kMcChunk - 1) / kMcChunk), dim3(256), 0, s,
-                           table, npics, preds, n, out, out_bytes, status);
+        hipLaunchKernelGGL(mc_luma_kernel,
+                           dim3(std::max((n + kMcChunk - 1) / kMcChunk, std::min(n, 8 * device_cus()))),
+                           dim3(256), 0, s, table, npics, preds, n, out, out_bytes, status);
     return hipGetLastError();
 }
 
