@@ -45,12 +45,12 @@ def test_gpu_luma_mc_matches_oracle(engine, orc, shape):
 
 @pytest.mark.parametrize("n", [8, 16])
 def test_gpu_luma_mc_runs_of_small_blocks(engine, orc, n):
-    """Runs of 8x8 or 16x16 blocks (as a frame's call list holds them),
-    predicted eight to a wavefront: every phase, integer MVs, groups whose
-    blocks are all in the interior, groups with one block on or past the
-    interior's edge (the whole group then takes the per-block path), lists
-    whose length is not a multiple of eight, and a rejected descriptor inside
-    a group."""
+    """Runs of 8x8 or 16x16 blocks (as a frame's call list holds them), many
+    to a slot: every phase, integer MVs, runs whose blocks are all in the
+    interior, runs with one block on or past the interior's edge (its slot
+    then stages with clamped loads and fixes that block's border samples),
+    lists whose length is not a multiple of a slot, and a rejected
+    descriptor inside a run."""
     from tcodec_amd import me, synth
     print(f"luma mc runs of {n}x{n} blocks", flush=True)
     h, w = 240, 320
@@ -86,8 +86,8 @@ def test_gpu_luma_mc_runs_of_small_blocks(engine, orc, n):
 
 def test_gpu_luma_mc_interior_blocks(engine, orc):
     """Blocks whose every sample has its 6x6 apron inside the frame (the
-    kernel's batched path), each of the 16 phases, block sizes that do and do
-    not divide a wavefront, next to blocks on the edges of that region."""
+    kernel's staged slots; sizes other than 8/16/32/64 sample by sample),
+    each of the 16 phases, next to blocks on the edges of that region."""
     from tcodec_amd import synth
     print("luma mc interior 640x360", flush=True)
     h, w = 360, 640
@@ -309,3 +309,37 @@ def test_gpu_chroma_mc_interior_column_tasks(engine, orc):
         want = orc.inter_predict_chroma(pl, int(mvs[k, 0]), int(mvs[k, 1]), n)
         assert np.array_equal(out[o:o + n * n].reshape(n, n), want), (k, mvs[k], n)
     engine.free(cp)
+
+
+def test_gpu_luma_mc_runs_of_one_phase(engine, orc):
+    """Runs of 8x8 and 16x16 blocks sharing one quarter-sample phase, for each
+    of the 16 phases: the kernel sorts a workgroup's blocks by the work their
+    phase needs (dx even, dy zero) and gives a slot whose blocks all agree
+    the task code of that kind, so these runs exercise every kind on full
+    slots, interior blocks and blocks on the picture's edges alike."""
+    from tcodec_amd import synth
+    h, w = 144, 208
+    print(f"luma mc runs of one phase {w}x{h}", flush=True)
+    ref = synth.Scene(w, h, seed=31, rects=9).frame(2)[0]
+    rp = engine.pic(ref)
+    rng = np.random.default_rng(77)
+    sizes, mvs = [], []
+    for ph in range(16):
+        for n, count in ((8, 96), (16, 40)):
+            fx = rng.integers(-4, w - n + 4, count)
+            fy = rng.integers(-4, h - n + 4, count)
+            fx[: count // 2] = rng.integers(2, w - n - 3, count // 2)   # interior half
+            fy[: count // 2] = rng.integers(2, h - n - 3, count // 2)
+            sizes += [n] * count
+            mvs += list(zip(fx * 4 + ph % 4, fy * 4 + ph // 4))
+    sizes, mvs = np.array(sizes), np.array(mvs)
+    preds = _preds(rp, mvs, sizes)
+    out, st = engine.predict(preds)
+    assert (st == 0).all()
+    for k in range(len(preds)):
+        n = int(sizes[k])
+        o = int(preds["out_off"][k])
+        got = out[o:o + n * n].reshape(n, n)
+        want = orc.inter_predict(ref, int(mvs[k, 0]), int(mvs[k, 1]), n)
+        assert np.array_equal(got, want), (k, mvs[k], n)
+    engine.free(rp)
