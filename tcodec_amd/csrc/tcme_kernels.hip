@@ -1529,7 +1529,7 @@ struct GroupWalk {
 // short candidate chains; 16 calls of two rows each per wavefront measured
 // 0.5% slower, 32 of four rows 5% slower).
 template <int N> struct Lanes {
-    static constexpr int L = N == 64 ? 64 : N == 32 ? 16 : 8;
+    static constexpr int L = N == 64 ? 64 : N == 32 ? 32 : 8;
     static constexpr int C = 64 / L;  // calls per slot
 };
 __host__ __device__ constexpr int size_class(int n) { return n == 64 ? 0 : n == 32 ? 1 : n == 16 ? 2 : 3; }
