@@ -285,10 +285,9 @@ struct McTask {
 // region and return their task; `edge` tells whether a block of the slot
 // has an apron reaching past the picture.
 //   no such block: 16-byte loads;
-//   else: dword loads with the row clamped into the picture and the column
-//         into the pitch (a clamped dword lies wholly outside the picture,
-//         as the columns are dword-aligned, so every in-picture byte of the
-//         copy is exact).
+//   else: dword loads with the row clamped into the picture and the dword
+//         into the row, its bytes permuted so that the copy's byte at
+//         column X is the plane's at clamp(X, 0, w - 1).
 template <int N>
 __device__ __forceinline__ McTask mc_stage(const McInfo *info, const uint8_t *idx, int cnt,
                                            int band, uint8_t *lds, int lane, McInfo &m, bool &edge)
@@ -329,7 +328,7 @@ __device__ __forceinline__ McTask mc_stage(const McInfo *info, const uint8_t *id
             const int s = min(li + S::L * i, DS - 1);
             const int row = s / D4, dc = s % D4;
             const int yy = min(max(m.fy - 2 + rb + row, 0), m.h - 1);
-            const int xx = min(max(xs + 4 * dc, 0), m.pitch - 4);
+            const int xx = min(max(xs + 4 * dc, 0), max(m.w - 4, 0));
             v[i] = *(const __attribute__((address_space(1))) uint32_t *)(
                 src + static_cast<size_t>(yy) * m.pitch + xx);
         }
@@ -337,7 +336,16 @@ __device__ __forceinline__ McTask mc_stage(const McInfo *info, const uint8_t *id
         for (int i = 0; i < DPL; ++i) {
             const int s = min(li + S::L * i, DS - 1);
             const int row = s / D4, dc = s % D4;
-            *reinterpret_cast<uint32_t *>(blds + row * S::P + 4 * dc) = v[i];
+            // byte j of the copy is the plane's byte at column
+            // clamp(x0 + j, 0, w - 1), which lies in the loaded dword
+            const int x0 = xs + 4 * dc;
+            const int xx = min(max(x0, 0), max(m.w - 4, 0));
+            uint32_t sel = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                sel |= static_cast<uint32_t>(min(max(x0 + j, 0), m.w - 1) - xx) << (8 * j);
+            *reinterpret_cast<uint32_t *>(blds + row * S::P + 4 * dc) =
+                __builtin_amdgcn_perm(0u, v[i], sel);
         }
     }
     McTask t;
@@ -355,11 +363,9 @@ __device__ __forceinline__ McTask mc_stage(const McInfo *info, const uint8_t *id
 // one 16-bit store when aligned).  The slot's blocks share one code path:
 // the specialised one of their kind when they all have the same (mc_kind),
 // else the general one.  `edge` (the slot's blocks may reach past the
-// picture): a sample whose 6x6 window is in the picture keeps the six-tap
-// value, any other one is stored again with the ladder's bilinear (2x2 in
-// the picture) or nearest value from the LDS copy (in-picture bytes are
-// exact there), or from the plane when the nearest sample lies outside the
-// copy; the same lane stores both, in order.
+// picture, staged with clamped coordinates): a sample whose 6x6 window is
+// in the picture keeps the six-tap value, any other one takes the ladder's
+// bilinear (2x2 in the picture) or nearest value, both read from the copy.
 __device__ __forceinline__ void mc_compute(const McTask &t, const McInfo &m, bool edge)
 {
     // the wavefront's LDS writes before its reads (one wavefront owns the region)
@@ -379,6 +385,47 @@ __device__ __forceinline__ void mc_compute(const McTask &t, const McInfo &m, boo
         default: mc_task<0>(t.lds, t.P, a, dx, dy, t.c, t.r0, res); break;
         }
         const gout8 oc = (gout8)m.dst + static_cast<int64_t>(t.rb + t.r0) * m.ostride + t.c;
+        if (edge && (m.frac & 64)) {
+            // samples whose 6x6 window leaves the picture (bit 2r + column)
+            // take the ladder's bilinear value (2x2 in the picture) or its
+            // nearest sample, both from the copy, which holds the plane with
+            // clamped coordinates; patched into the packed results
+            const int x = m.fx + t.c, y = m.fy + t.rb + t.r0;
+            uint32_t fix = 0, bil = 0;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                    const int xk = x + cc, yk = y + r;
+                    if (!(xk >= 2 && xk + 3 < m.w && yk >= 2 && yk + 3 < m.h))
+                        fix |= 1u << (2 * r + cc);
+                    if (xk >= 0 && xk + 1 < m.w && yk >= 0 && yk + 1 < m.h)
+                        bil |= 1u << (2 * r + cc);
+                }
+            }
+            if (fix) {
+                // rows y .. y+8, columns x .. x+2 of the copy, read together
+                const uint8_t *q = t.lds + (t.r0 + 2) * t.P + t.c + a + 2;
+                int p[9][3];
+#pragma unroll
+                for (int i = 0; i < 9; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) p[i][j] = q[i * t.P + j];
+                const int ox = dx >= 2 ? 1 : 0, oy = dy >= 2 ? 1 : 0;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const int r = k >> 1, cc = k & 1;
+                    const int b = lerp4(lerp4(p[r][cc], p[r][cc + 1], dx),
+                                        lerp4(p[r + 1][cc], p[r + 1][cc + 1], dx), dy);
+                    const int nr = oy ? p[r + 1][cc] : p[r][cc];
+                    const int nn = oy ? p[r + 1][cc + 1] : p[r][cc + 1];
+                    const int v = ((bil >> k) & 1) ? b : (ox ? nn : nr);
+                    if ((fix >> k) & 1)
+                        res[r] = cc ? (res[r] & 0xffffu) | (static_cast<uint32_t>(v) << 16)
+                                    : (res[r] & 0xffff0000u) | static_cast<uint32_t>(v);
+                }
+            }
+        }
         if (((m.dst + t.c) & 1) == 0 && (m.ostride & 1) == 0) {
             // both columns of a row in one 16-bit store
             typedef __attribute__((address_space(1))) uint16_t *gout16;
@@ -391,39 +438,6 @@ __device__ __forceinline__ void mc_compute(const McTask &t, const McInfo &m, boo
             for (int r = 0; r < 8; ++r) {
                 oc[static_cast<int64_t>(r) * m.ostride] = static_cast<uint8_t>(res[r]);
                 oc[static_cast<int64_t>(r) * m.ostride + 1] = static_cast<uint8_t>(res[r] >> 16);
-            }
-        }
-        if (edge && (m.frac & 64)) {
-            // samples whose 6x6 window leaves the picture (bit 2r + column),
-            // stored again with the ladder's value
-            const int x = m.fx + t.c, y = m.fy + t.rb + t.r0;
-            uint32_t colok = 0, rowok = 0;
-#pragma unroll
-            for (int cc = 0; cc < 2; ++cc)
-                if (x + cc >= 2 && x + cc + 3 < m.w) colok |= 0x5555u << cc;
-#pragma unroll
-            for (int r = 0; r < 8; ++r)
-                if (y + r >= 2 && y + r + 3 < m.h) rowok |= 3u << (2 * r);
-            uint32_t fix = ~(colok & rowok) & 0xffffu;
-            const int xs = (m.fx - 2) & ~3, ys = m.fy - 2 + t.rb;
-            while (fix) {
-                const int k = __builtin_ctz(fix);
-                fix &= fix - 1;
-                const int r = k >> 1, cc = k & 1;
-                const int xk = x + cc, yk = y + r;
-                int v;
-                if (xk >= 0 && yk >= 0 && xk + 1 < m.w && yk + 1 < m.h) {
-                    const uint8_t *q = t.lds + (yk - ys) * t.P + (xk - xs);
-                    v = lerp4(lerp4(q[0], q[1], dx), lerp4(q[t.P], q[t.P + 1], dx), dy);
-                } else {
-                    const int nx = min(max(xk + (dx >= 2 ? 1 : 0), 0), m.w - 1);
-                    const int ny = min(max(yk + (dy >= 2 ? 1 : 0), 0), m.h - 1);
-                    const int X = nx - xs, Y = ny - ys;
-                    v = X >= 0 && X < t.P && Y >= 0 && Y < t.rows
-                            ? t.lds[Y * t.P + X]
-                            : ((gptr8)m.src)[static_cast<size_t>(ny) * m.pitch + nx];
-                }
-                oc[static_cast<int64_t>(r) * m.ostride + cc] = static_cast<uint8_t>(v);
             }
         }
     }
@@ -448,7 +462,7 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
                                                       uint8_t *__restrict__ out, int64_t out_bytes,
                                                       int32_t *__restrict__ status)
 {
-    // classes: 0-3 sizes 64/32/16/8, 4 other sizes; keys: edge << 2 | kind
+    // classes: 0-3 sizes 64/32/16/8, 4 other sizes; keys: inner << 2 | kind
     __shared__ McInfo info[kMcChunk];
     __shared__ uint8_t lists[5][kMcChunk];
     __shared__ int counts[5];
@@ -501,7 +515,7 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
             m.h = R.h;
             info[tid] = m;
             cl = sz;
-            key = (inner ? 0 : 4) | mc_kind(p.mv_x & 3, p.mv_y & 3);
+            key = (inner ? 4 : 0) | mc_kind(p.mv_x & 3, p.mv_y & 3);
             rank = atomicAdd(&kcount[sz][key], 1);
         }
     }
@@ -544,7 +558,8 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
     constexpr int G32 = McShape<32>::G, G16 = McShape<16>::G, G8 = McShape<8>::G;
     const int n64 = counts[0], n32 = counts[1], n16 = counts[2], n8 = counts[3];
     // slots in the order they are taken: other sizes, 64 (4 bands a block),
-    // 32, 16, 8
+    // 32, 16, 8; within a size the blocks that reach past the picture, which
+    // cost the most, lead the list
     const int so = no ? oslot[no] : 0, s64 = 4 * n64, s32 = (n32 + G32 - 1) / G32,
               s16 = (n16 + G16 - 1) / G16, s8 = (n8 + G8 - 1) / G8;
     const int slots = so + s64 + s32 + s16 + s8;
