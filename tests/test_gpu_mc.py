@@ -343,3 +343,37 @@ def test_gpu_luma_mc_runs_of_one_phase(engine, orc):
         want = orc.inter_predict(ref, int(mvs[k, 0]), int(mvs[k, 1]), n)
         assert np.array_equal(got, want), (k, mvs[k], n)
     engine.free(rp)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 3), (7, 5), (9, 2), (4, 11), (13, 7), (3, 70)])
+def test_gpu_mc_pictures_smaller_than_the_block(engine, orc, shape):
+    """Pictures narrower or shorter than 4 samples and smaller than the
+    block: every block reaches past the picture, every sample clamps
+    (luma: 1..64 at every phase; chroma: 1..32 at every eighth)."""
+    h, w = shape
+    print(f"mc pictures smaller than the block {w}x{h}", flush=True)
+    rng = np.random.default_rng(h * 100 + w)
+    pl = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    rp, cp = engine.pic(pl), engine.plane(pl)
+    sizes = rng.choice([1, 2, 4, 8, 12, 16, 32, 64], 300)
+    mvs = np.stack([rng.integers(-4 * 80, 4 * (w + 12), 300), rng.integers(-4 * 80, 4 * (h + 12), 300)], 1)
+    preds = _preds(rp, mvs, sizes)
+    out, st = engine.predict(preds)
+    assert (st == 0).all()
+    for k in range(len(preds)):
+        n = int(sizes[k])
+        o = int(preds["out_off"][k])
+        want = orc.inter_predict(pl, int(mvs[k, 0]), int(mvs[k, 1]), n)
+        assert np.array_equal(out[o:o + n * n].reshape(n, n), want), (k, mvs[k], n)
+    csz = rng.choice([1, 2, 4, 8, 16, 32], 300)
+    cmv = np.stack([rng.integers(-8 * 40, 8 * (w + 12), 300), rng.integers(-8 * 40, 8 * (h + 12), 300)], 1)
+    preds = _preds(cp, cmv, csz)
+    out, st = engine.predict(preds, chroma=True)
+    assert (st == 0).all()
+    for k in range(len(preds)):
+        n = int(csz[k])
+        o = int(preds["out_off"][k])
+        want = orc.inter_predict_chroma(pl, int(cmv[k, 0]), int(cmv[k, 1]), n)
+        assert np.array_equal(out[o:o + n * n].reshape(n, n), want), (k, cmv[k], n)
+    engine.free(rp)
+    engine.free(cp)
