@@ -662,39 +662,109 @@ __device__ __forceinline__ void chroma_block(const PicDev *__restrict__ table, i
         chroma_rows<1>(R, xi, yi, fx, fy, o, p.out_stride, bl.c, bl.r0, bl.dr, p.blk);
 }
 
-// Rows [r0, r0 + 4) of column c of a chroma block whose every sample's 2x2
-// neighbourhood is inside the plane (no clamping), plane y / pitch, at
-// 1/8-sample (mvx, mvy): the two samples (x, x+1) of the 5 rows are loaded
-// first.
-__device__ __forceinline__ void chroma_col4(const uint8_t *y, int pitch, int mvx, int mvy, int c,
-                                            int r0, uint8_t *o, int os)
+// Rows [r0, r0 + 4) x columns [c, c + 4) of a chroma block, plane y / pitch
+// (w x h), at 1/8-sample (mvx, mvy).  Each of the 5 source rows is one
+// 8-byte load from a dword boundary (the plane's rows start 64-byte aligned
+// and end in padding or tail slack).  Without clamping (every sample's 2x2
+// neighbourhood inside the plane) the load starts at the dword below column
+// x and a 64-bit shift brings column x to byte 0; with clamping the row is
+// clamped into the plane, the load starts at the dword below clamp(x), and
+// a byte permute picks the bytes at clamp(x + j, 0, w - 1), all within the
+// 8 loaded.  A sample is two unsigned byte dot products (v_dot4_u32_u8) of
+// the rows' bytes against the weights (8-fx)(8-fy), fx(8-fy) and (8-fx)fy,
+// fx fy placed at its columns, the rounding 32 carried in, and a row's four
+// samples go out as one dword when the output is 4-byte aligned.
+__device__ __forceinline__ void chroma_tile(const uint8_t *y, int pitch, int w, int h, bool clamp,
+                                            int mvx, int mvy, int c, int r0, uint8_t *o, int os)
 {
+    typedef unsigned int u32x2a4 __attribute__((ext_vector_type(2), aligned(4)));
+    typedef const __attribute__((address_space(1))) u32x2a4 *gptr64u;
     const int xi = mvx >> 3, yi = mvy >> 3, fx = mvx & 7, fy = mvy & 7;
-    const gptr8 p = (gptr8)y + static_cast<size_t>(yi + r0) * pitch + xi + c;
-    int a[5], b[5];
+    const int x = xi + c;
+    uint32_t lo[5], l1[5];  // bytes x .. x+3 and x+1 .. x+4 of each row
+    if (!clamp) {
+        const int sh = x & 3;
+        const gptr8 rp = (gptr8)y + static_cast<size_t>(yi + r0) * pitch + (x & ~3);
+        uint64_t raw[5];
 #pragma unroll
-    for (int r = 0; r < 5; ++r) {
-        a[r] = p[static_cast<size_t>(r) * pitch];
-        b[r] = p[static_cast<size_t>(r) * pitch + 1];
+        for (int r = 0; r < 5; ++r) {
+            const u32x2a4 d = *(gptr64u)(rp + static_cast<size_t>(r) * pitch);
+            raw[r] = static_cast<uint64_t>(d.x) | (static_cast<uint64_t>(d.y) << 32);
+        }
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const uint64_t t = raw[r] >> (8 * sh);
+            lo[r] = static_cast<uint32_t>(t);
+            l1[r] = static_cast<uint32_t>(t >> 8);
+        }
+    } else {
+        const int base = min(max(x, 0), w - 1) & ~3;
+        uint32_t sel = 0, sel1 = 0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const uint32_t sj = static_cast<uint32_t>(min(max(x + j, 0), w - 1) - base);  // 0..7
+            if (j < 4) sel |= sj << (8 * j);
+            if (j > 0) sel1 |= sj << (8 * (j - 1));
+        }
+        u32x2a4 d[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const int yy = min(max(yi + r0 + r, 0), h - 1);
+            d[r] = *(gptr64u)((gptr8)y + static_cast<size_t>(yy) * pitch + base);
+        }
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            lo[r] = __builtin_amdgcn_perm(d[r].y, d[r].x, sel);
+            l1[r] = __builtin_amdgcn_perm(d[r].y, d[r].x, sel1);
+        }
     }
-    const int w00 = (8 - fx) * (8 - fy), w10 = fx * (8 - fy), w01 = (8 - fx) * fy, w11 = fx * fy;
+    const uint32_t wa = static_cast<uint32_t>((8 - fx) * (8 - fy)) | (static_cast<uint32_t>(fx * (8 - fy)) << 8);
+    const uint32_t wc = static_cast<uint32_t>((8 - fx) * fy) | (static_cast<uint32_t>(fx * fy) << 8);
     const gout8 oc = (gout8)o + static_cast<int64_t>(r0) * os + c;
+    const bool al = ((reinterpret_cast<uint64_t>(o) + c) & 3) == 0 && (os & 3) == 0;
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-        oc[static_cast<int64_t>(r) * os] = static_cast<uint8_t>(
-            (w00 * a[r] + w10 * b[r] + w01 * a[r + 1] + w11 * b[r + 1] + 32) >> 6);
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t v0 = __builtin_amdgcn_udot4(lo[r], wa, __builtin_amdgcn_udot4(lo[r + 1], wc, 32u, false), false) >> 6;
+        const uint32_t v1 = __builtin_amdgcn_udot4(lo[r], wa << 8, __builtin_amdgcn_udot4(lo[r + 1], wc << 8, 32u, false), false) >> 6;
+        const uint32_t v2 = __builtin_amdgcn_udot4(lo[r], wa << 16, __builtin_amdgcn_udot4(lo[r + 1], wc << 16, 32u, false), false) >> 6;
+        const uint32_t v3 = __builtin_amdgcn_udot4(l1[r], wa << 16, __builtin_amdgcn_udot4(l1[r + 1], wc << 16, 32u, false), false) >> 6;
+        const gout8 orow = oc + static_cast<int64_t>(r) * os;
+        if (al) {
+            *(__attribute__((address_space(1))) uint32_t *)orow = v0 | (v1 << 8) | (v2 << 16) | (v3 << 24);
+        } else {
+            orow[0] = static_cast<uint8_t>(v0);
+            orow[1] = static_cast<uint8_t>(v1);
+            orow[2] = static_cast<uint8_t>(v2);
+            orow[3] = static_cast<uint8_t>(v3);
+        }
+    }
 }
 
 // Chroma at 1/8 sample (tc_inter_predict_chroma, src/motion.c:597-632), the
 // same work split as the luma kernel: a chunk of the list per workgroup,
-// taken 64 descriptors at a time; blocks of 4x4 .. 32x32 whose samples need
-// no edge clamping are cut into tasks of one column x 4 rows (m * m / 4 per
-// block), numbered in a row over the 64 descriptors and taken by the
-// workgroup's 256 lanes 256 at a time (chroma_col4: one round trip to memory
-// per pass); other blocks are predicted one per wavefront, in turn.  (Quads
-// of consecutive blocks per wavefront, four 4x4 blocks at 16 lanes each and
-// larger ones one after another, took 63 us for a 1080p frame's 343,200
-// blocks, most of it in the 16x16 and 32x32 blocks' rounds of loads.)
+// taken 256 descriptors at a time (64 a wavefront); blocks of 4x4 .. 32x32
+// (with or without edge clamping) are cut into tasks of 4 x 4 samples
+// (m * m / 16 per block), numbered in a row over the 256 descriptors (a
+// prefix sum over each wavefront's lanes, then over the wavefronts) and
+// taken by the workgroup's 256 lanes 256 at a time, each finding its block
+// by a binary search over the prefix sums in LDS (chroma_tile: one round trip
+// to memory per pass); blocks of other sizes are predicted one per wavefront,
+// each wavefront taking those of its own 64 descriptors.  (Before: tasks of one
+// column x 4 rows from byte loads, numbered over 64 descriptors, 26 us for a
+// 1080p frame's 343,200 blocks; 4 x 4 tasks numbered over 64 descriptors,
+// 21.7 us, a run of 4x4 blocks then filling one wavefront of the four.
+// Quads of consecutive blocks per wavefront, four 4x4 blocks at 16 lanes
+// each and larger ones one after another, took 63 us, most of it in the
+// 16x16 and 32x32 blocks' rounds of loads.)
+struct ChromaTask {
+    uint64_t y, o;   // plane, block's output
+    int mvx, mvy;
+    int pitch, os;
+    int w, h;        // plane size
+    int kl;          // log2 of the block size
+    int clamp;       // a sample's 2x2 neighbourhood may leave the plane
+};
+
 __global__ __launch_bounds__(256) void mc_chroma_kernel(const PicDev *__restrict__ table,
                                                         int npics,
                                                         const tcme_pred *__restrict__ preds,
@@ -702,8 +772,11 @@ __global__ __launch_bounds__(256) void mc_chroma_kernel(const PicDev *__restrict
                                                         int64_t out_bytes,
                                                         int32_t *__restrict__ status)
 {
-    const int lane = static_cast<int>(threadIdx.x & 63);
-    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    __shared__ ChromaTask info[256];
+    __shared__ int excl_s[256];      // first task of each descriptor
+    __shared__ int wtot[4];          // tasks per wavefront
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     constexpr int kXcd = 8;
     const int g = static_cast<int>(gridDim.x), bid = static_cast<int>(blockIdx.x);
     const int q = g / kXcd, rr = g % kXcd, x = bid % kXcd;
@@ -712,58 +785,72 @@ __global__ __launch_bounds__(256) void mc_chroma_kernel(const PicDev *__restrict
     const int64_t c1 = static_cast<int64_t>(n) * (chunk + 1) / g;
     PicDev R{};
     int cached = -1;
-    for (int64_t w0 = c0; w0 < c1; w0 += 64) {
-        const int cnt = c1 - w0 < 64 ? static_cast<int>(c1 - w0) : 64;
+    for (int64_t w0 = c0; w0 < c1; w0 += 256) {
+        const int64_t i = w0 + tid;  // this lane's descriptor
         tcme_pred p{};
         bool ok = false;
-        uint64_t y = 0;
-        int pitch = 0, lg = 0;
-        if (lane < cnt) {
-            p = preds[w0 + lane];
-            lg = p.blk == 4 ? 2 : p.blk == 8 ? 3 : p.blk == 16 ? 4 : p.blk == 32 ? 5 : 0;
+        ChromaTask ct{};
+        if (i < c1) {
+            p = preds[i];
+            const int lg = p.blk == 4 ? 2 : p.blk == 8 ? 3 : p.blk == 16 ? 4 : p.blk == 32 ? 5 : 0;
             if (lg && p.pic >= 0 && p.pic < npics) {
                 const PicDev Rk = table[p.pic];
                 const int xi = p.mv_x >> 3, yi = p.mv_y >> 3;
-                ok = Rk.y && span_fits(p.out_off, p.blk, p.blk, p.out_stride, out_bytes) &&
-                     xi >= 0 && yi >= 0 && xi + p.blk + 1 <= Rk.w && yi + p.blk + 1 <= Rk.h;
-                y = reinterpret_cast<uint64_t>(Rk.y);
-                pitch = Rk.pitch;
+                ok = Rk.y && span_fits(p.out_off, p.blk, p.blk, p.out_stride, out_bytes);
+                ct.clamp = !(xi >= 0 && yi >= 0 && xi + p.blk + 1 <= Rk.w && yi + p.blk + 1 <= Rk.h);
+                ct.w = Rk.w;
+                ct.h = Rk.h;
+                ct.y = reinterpret_cast<uint64_t>(Rk.y);
+                ct.o = reinterpret_cast<uint64_t>(out + p.out_off);
+                ct.mvx = p.mv_x;
+                ct.mvy = p.mv_y;
+                ct.pitch = Rk.pitch;
+                ct.os = p.out_stride;
+                ct.kl = lg;
             }
         }
-        const int tasks = ok ? p.blk * p.blk / 4 : 0;
-        int incl = tasks;  // inclusive prefix sum over the lanes
+        const int tasks = ok ? p.blk * p.blk / 16 : 0;
+        int incl = tasks;  // inclusive prefix sum over the wavefront's lanes
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
             const int t = shfl32(incl, lane >= d ? lane - d : lane);
             if (lane >= d) incl += t;
         }
-        const int excl = incl - tasks;
-        const int total = __builtin_amdgcn_readlane(incl, 63);
-        if (status && ok && wave == 0) status[w0 + lane] = 0;
-        for (int t0 = wave * 64; t0 < total; t0 += 256) {
-            const int t = t0 + lane, tt = t < total ? t : total - 1;
-            int k = 0;  // the last descriptor whose tasks start at or before tt
+        if (lane == 63) wtot[wave] = incl;
+        if (status && ok) status[i] = 0;
+        info[tid] = ct;
+        __syncthreads();
+        int before = 0, total = 0;
 #pragma unroll
-            for (int sft = 32; sft; sft >>= 1) {
-                const int e = shfl32(excl, k + sft < 64 ? k + sft : 63);
-                if (k + sft < cnt && e <= tt) k += sft;
-            }
-            const int u = tt - shfl32(excl, k), kl = shfl32(lg, k);
-            const int mvx = shfl32(p.mv_x, k), mvy = shfl32(p.mv_y, k);
-            const int os = shfl32(p.out_stride, k), pt = shfl32(pitch, k);
-            const uint64_t yy = shfl64(y, k), oo = shfl64(static_cast<uint64_t>(p.out_off), k);
-            if (t < total)
-                chroma_col4(reinterpret_cast<const uint8_t *>(yy), pt, mvx, mvy,
-                            u & ((1 << kl) - 1), (u >> kl) * 4, out + oo, os);
+        for (int k = 0; k < 4; ++k) {
+            const int v = wtot[k];
+            if (k < wave) before += v;
+            total += v;
         }
-        uint64_t m = __ballot(lane < cnt && !ok);
-        for (int j = 0; m; ++j) {
+        excl_s[tid] = before + incl - tasks;
+        __syncthreads();
+        for (int t = tid; t < total; t += 256) {
+            int k = 0;  // the last descriptor whose tasks start at or before t
+            // (k + sft <= 256 - sft at every step: the read stays in the array)
+#pragma unroll
+            for (int sft = 128; sft; sft >>= 1)
+                if (excl_s[k + sft] <= t) k += sft;
+            const ChromaTask c = info[k];
+            const int u = t - excl_s[k], kl = c.kl;
+            // tile u of the block: (u mod m/4, u div m/4) in units of 4 samples
+            chroma_tile(reinterpret_cast<const uint8_t *>(c.y), c.pitch, c.w, c.h, c.clamp != 0,
+                        c.mvx, c.mvy,
+                        4 * (u & ((1 << (kl - 2)) - 1)), 4 * (u >> (kl - 2)),
+                        reinterpret_cast<uint8_t *>(c.o), c.os);
+        }
+        uint64_t m = __ballot(i < c1 && !ok);
+        while (m) {
             const int k = __builtin_ctzll(m);
             m &= m - 1;
-            if ((j & 3) == wave)
-                chroma_block(table, npics, preds[w0 + k], w0 + k, out, out_bytes, status, R,
-                             cached, lane);
+            const int64_t j = w0 + 64 * wave + k;
+            chroma_block(table, npics, preds[j], j, out, out_bytes, status, R, cached, lane);
         }
+        __syncthreads();  // info and excl_s are rewritten by the next round
     }
 }
 
