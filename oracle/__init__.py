@@ -146,6 +146,19 @@ class Oracle:
                                           n, n)
         return dst
 
+    def inter_predict_list(self, ref, mvs, sizes, offs, out, chroma=False):
+        """Every block k (MV mvs[k], size sizes[k], packed at out[offs[k]:]
+        with row stride sizes[k]) into the 1-D uint8 array `out`."""
+        f = self.lib.orc_inter_predict_chroma if chroma else self.lib.orc_inter_predict
+        h, w = ref.shape
+        rp, st, base = ptr(ref), ref.strides[0], out.ctypes.data
+        assert out.dtype == np.uint8 and out.ndim == 1 and out.flags.c_contiguous
+        for k in range(len(sizes)):
+            n, o = int(sizes[k]), int(offs[k])
+            assert 0 <= o and o + n * n <= out.size
+            f(rp, st, w, h, int(mvs[k][0]), int(mvs[k][1]), C.cast(base + o, u8p), n, n)
+        return out
+
     def mvp(self, cx, cy, dx, dy, intra):
         ox, oy = C.c_int32(), C.c_int32()
         dx = np.ascontiguousarray(dx, np.int16)

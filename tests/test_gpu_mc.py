@@ -45,12 +45,13 @@ def test_gpu_luma_mc_matches_oracle(engine, orc, shape):
 
 @pytest.mark.parametrize("n", [8, 16])
 def test_gpu_luma_mc_runs_of_small_blocks(engine, orc, n):
-    """Runs of 8x8 or 16x16 blocks (as a frame's call list holds them), many
-    to a slot: every phase, integer MVs, runs whose blocks are all in the
-    interior, runs with one block on or past the interior's edge (its slot
-    then stages with clamped loads and fixes that block's border samples),
-    lists whose length is not a multiple of a slot, and a rejected
-    descriptor inside a run."""
+    """Runs of 8x8 or 16x16 blocks (as a frame's call list holds them): every
+    phase, integer MVs, runs whose blocks are all in the interior, runs with
+    one block on or past the interior's edge (its slot then stages with
+    clamped loads and fixes that block's border samples), and a rejected
+    descriptor inside a run.  (The launch spreads a list over at least
+    2,048 workgroups, so at these lengths a workgroup holds one or two
+    blocks; test_gpu_luma_mc_frame_sized_list fills the slots.)"""
     from tcodec_amd import me, synth
     print(f"luma mc runs of {n}x{n} blocks", flush=True)
     h, w = 240, 320
@@ -315,8 +316,10 @@ def test_gpu_luma_mc_runs_of_one_phase(engine, orc):
     """Runs of 8x8 and 16x16 blocks sharing one quarter-sample phase, for each
     of the 16 phases: the kernel sorts a workgroup's blocks by the work their
     phase needs (dx even, dy zero) and gives a slot whose blocks all agree
-    the task code of that kind, so these runs exercise every kind on full
-    slots, interior blocks and blocks on the picture's edges alike."""
+    the task code of that kind, so these runs exercise every kind, interior
+    blocks and blocks on the picture's edges alike (one or two blocks a
+    workgroup at this length; the frame-sized list mixes kinds in full
+    slots)."""
     from tcodec_amd import synth
     h, w = 144, 208
     print(f"luma mc runs of one phase {w}x{h}", flush=True)
@@ -377,3 +380,97 @@ def test_gpu_mc_pictures_smaller_than_the_block(engine, orc, shape):
         assert np.array_equal(out[o:o + n * n].reshape(n, n), want), (k, cmv[k], n)
     engine.free(rp)
     engine.free(cp)
+
+
+def _ctu_lists(rng, w, h, passes, reach, blk_scale=1):
+    """Block lists as an encoder issues them: CTU after CTU, each CTU's
+    quadtree nodes (1 64x64, 4 32x32, 16 16x16, 64 8x8, sizes divided by
+    blk_scale), each pass over the picture with fresh MVs (block position
+    plus up to +-reach samples, every fraction)."""
+    sub = 8 // blk_scale  # fractions per sample: 4 luma, 8 chroma
+    ctu = 64 // blk_scale
+    nodes = [(n // blk_scale, bx, by) for n in (64, 32, 16, 8)
+             for by in range(0, ctu, n // blk_scale) for bx in range(0, ctu, n // blk_scale)]
+    ns, nx, ny = (np.array(v) for v in zip(*nodes))
+    cx, cy = np.meshgrid(np.arange(0, w, ctu), np.arange(0, h, ctu))
+    cx, cy = cx.ravel(), cy.ravel()
+    sizes = np.tile(ns, len(cx) * passes)
+    px = np.tile((cx[:, None] + nx[None, :]).ravel(), passes)
+    py = np.tile((cy[:, None] + ny[None, :]).ravel(), passes)
+    k = len(sizes)
+    mvs = np.stack([(px + rng.integers(-reach, reach + 1, k)) * sub + rng.integers(0, sub, k),
+                    (py + rng.integers(-reach, reach + 1, k)) * sub + rng.integers(0, sub, k)], 1)
+    return sizes, mvs
+
+
+def test_gpu_luma_mc_frame_sized_list(engine, orc):
+    """A list as long as a 1080p frame's (142,800 blocks in CTU order over two
+    references): the launch gives each workgroup a chunk of 64 descriptors,
+    so its slots fill (16 8x8 blocks, 4 16x16, 32x32 and 64x64 bands), blocks
+    are sorted by kind with those reaching past the picture first, and runs
+    of other sizes and a rejected descriptor sit among them.  Every block
+    against the oracle."""
+    from tcodec_amd import synth
+    h, w = 720, 1280
+    print(f"luma mc frame-sized list {w}x{h}", flush=True)
+    sc = synth.Scene(w, h, seed=51, rects=16)
+    refs = [sc.frame(2)[0], sc.frame(5)[0]]
+    pics = [engine.pic(r) for r in refs]
+    rng = np.random.default_rng(52)
+    sizes, mvs = _ctu_lists(rng, w, h, 7, 24)
+    which = (np.arange(len(sizes)) // (len(sizes) // 7)) % 2
+    odd = rng.choice(len(sizes), 400, replace=False)
+    sizes[odd] = rng.choice([4, 12, 24, 40], 400)
+    preds = _preds(np.array(pics)[which], mvs, sizes)
+    preds["pic"][777] = 999
+    out, st = engine.predict(preds)
+    assert st[777] == -4 and (np.delete(st, 777) == 0).all()
+    want = np.zeros_like(out)
+    for r in (0, 1):
+        k = np.nonzero(which == r)[0]
+        orc.inter_predict_list(refs[r], mvs[k], sizes[k], preds["out_off"][k], want)
+    n2 = sizes.astype(np.int64) ** 2
+    offs = preds["out_off"]
+    want[offs[777]:offs[777] + n2[777]] = 0  # the rejected block is left untouched
+    if not np.array_equal(out, want):
+        bad = [k for k in range(len(sizes)) if not np.array_equal(
+            out[offs[k]:offs[k] + n2[k]], want[offs[k]:offs[k] + n2[k]])]
+        assert not bad, [(k, int(sizes[k]), tuple(mvs[k])) for k in bad[:8]]
+    for p in pics:
+        engine.free(p)
+
+
+def test_gpu_chroma_mc_frame_sized_list(engine, orc):
+    """A chroma list longer than 2,048 workgroups x 256 descriptors (652,800
+    blocks of 4x4 .. 32x32 in CTU order over both planes): every workgroup
+    takes two rounds of 256 descriptors, its 4x4 tasks numbered over all
+    four wavefronts; blocks needing clamps, other sizes and a rejected
+    descriptor among them.  Every block against the oracle."""
+    from tcodec_amd import synth
+    h, w = 720, 1280
+    print(f"chroma mc frame-sized list {w // 2}x{h // 2}", flush=True)
+    _, cb, cr = synth.Scene(w, h, seed=53, rects=16).frame(3)
+    planes = [cb, cr]
+    pics = [engine.plane(p) for p in planes]
+    rng = np.random.default_rng(54)
+    sizes, mvs = _ctu_lists(rng, w // 2, h // 2, 32, 20, blk_scale=2)
+    which = (np.arange(len(sizes)) // 85) % 2
+    odd = rng.choice(len(sizes), 300, replace=False)
+    sizes[odd] = rng.choice([1, 2, 3, 6], 300)
+    preds = _preds(np.array(pics)[which], mvs, sizes)
+    preds["pic"][4321] = 999
+    out, st = engine.predict(preds, chroma=True)
+    assert st[4321] == -4 and (np.delete(st, 4321) == 0).all()
+    want = np.zeros_like(out)
+    for r in (0, 1):
+        k = np.nonzero(which == r)[0]
+        orc.inter_predict_list(planes[r], mvs[k], sizes[k], preds["out_off"][k], want, chroma=True)
+    n2 = sizes.astype(np.int64) ** 2
+    offs = preds["out_off"]
+    want[offs[4321]:offs[4321] + n2[4321]] = 0  # the rejected block is left untouched
+    if not np.array_equal(out, want):
+        bad = [k for k in range(len(sizes)) if not np.array_equal(
+            out[offs[k]:offs[k] + n2[k]], want[offs[k]:offs[k] + n2[k]])]
+        assert not bad, [(k, int(sizes[k]), tuple(mvs[k])) for k in bad[:8]]
+    for p in pics:
+        engine.free(p)
