@@ -274,14 +274,14 @@ def test_gpu_mc_strided_output_and_bad_descriptors(engine, orc):
     engine.free(pp)
 
 
-def test_gpu_chroma_mc_interior_column_tasks(engine, orc):
+def test_gpu_chroma_mc_interior_tasks(engine, orc):
     """Blocks of 4x4 .. 32x32 whose every sample's 2x2 neighbourhood is inside
-    the plane (cut into column tasks shared by a workgroup's lanes), every
+    the plane (cut into 4x4 tasks shared by a workgroup's lanes), every
     1/8-sample fraction, in runs of one size and mixed as a CTU's quadtree
     levels are, next to blocks one sample past that region and other sizes,
     with a rejected descriptor among them."""
     from tcodec_amd import synth
-    print("chroma mc interior column tasks 320x240", flush=True)
+    print("chroma mc interior tasks 320x240", flush=True)
     h, w = 240, 320
     pl = synth.Scene(w, h, seed=21, rects=9).frame(2)[1]
     pl = np.ascontiguousarray(np.repeat(np.repeat(pl, 2, 0), 2, 1)[:h, :w])
