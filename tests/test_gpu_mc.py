@@ -474,3 +474,35 @@ def test_gpu_chroma_mc_frame_sized_list(engine, orc):
         assert not bad, [(k, int(sizes[k]), tuple(mvs[k])) for k in bad[:8]]
     for p in pics:
         engine.free(p)
+
+
+@pytest.mark.parametrize("stride,shift", [(77, 0), (78, 1), (80, 2), (83, 3)])
+def test_gpu_mc_unaligned_output(engine, orc, stride, shift):
+    """Blocks written with a row stride and a start that are not multiples
+    of 2 or 4 (the kernels then store byte by byte instead of 2 or 4
+    samples at a time), interior and clamped blocks alike; the bytes between
+    rows stay as they were."""
+    h, w = 136, 200
+    print(f"mc unaligned output stride {stride} shift {shift}", flush=True)
+    rng = np.random.default_rng(stride * 4 + shift)
+    ref = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    rp, cp = engine.pic(ref), engine.plane(ref)
+    for chroma, choices, sub in ((False, [8, 16, 32, 64, 12], 4), (True, [4, 8, 16, 32, 2], 8)):
+        count = 6000
+        sizes = rng.choice(choices, count)
+        mvs = np.stack([rng.integers(-12 * sub, (w + 4) * sub, count),
+                        rng.integers(-12 * sub, (h + 4) * sub, count)], 1)
+        p = _preds(cp if chroma else rp, mvs, sizes, stride=stride)
+        p["out_off"] = np.arange(count, dtype=np.int64) * stride * 64 + shift
+        out, st = engine.predict(p, chroma=chroma)
+        assert (st == 0).all()
+        f = orc.inter_predict_chroma if chroma else orc.inter_predict
+        gaps = np.ones(out.size, bool)
+        for k in range(count):
+            n, o = int(sizes[k]), int(p["out_off"][k])
+            rows = o + stride * np.arange(n)[:, None] + np.arange(n)[None, :]
+            assert np.array_equal(out[rows], f(ref, int(mvs[k, 0]), int(mvs[k, 1]), n)), (chroma, k, n, tuple(mvs[k]))
+            gaps[rows] = False
+        assert not out[gaps].any()
+    engine.free(rp)
+    engine.free(cp)
