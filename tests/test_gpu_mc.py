@@ -506,3 +506,30 @@ def test_gpu_mc_unaligned_output(engine, orc, stride, shift):
         assert not out[gaps].any()
     engine.free(rp)
     engine.free(cp)
+
+
+def test_gpu_mc_extreme_motion_vectors(engine, orc):
+    """MVs at the ends of the int32 range and around +-2^30 (every sample
+    clamps to a corner or an edge of the picture), all sizes, luma and
+    chroma."""
+    h, w = 50, 70
+    print("mc extreme motion vectors", flush=True)
+    rng = np.random.default_rng(99)
+    ref = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    rp, cp = engine.pic(ref), engine.plane(ref)
+    big = [-2 ** 31, -2 ** 31 + 5, -2 ** 30 - 3, -2 ** 20, 2 ** 20 + 1, 2 ** 30 + 2, 2 ** 31 - 1, 2 ** 31 - 6]
+    small = [0, 9, 4 * w - 3]
+    vals = big + small
+    for chroma, choices in ((False, [1, 8, 16, 32, 64, 24]), (True, [1, 4, 8, 16, 32, 3])):
+        mvs = np.array([(a, b) for a in vals for b in vals if a in big or b in big] * 2, np.int64)
+        sizes = rng.choice(choices, len(mvs))
+        p = _preds(cp if chroma else rp, mvs, sizes)
+        out, st = engine.predict(p, chroma=chroma)
+        assert (st == 0).all()
+        f = orc.inter_predict_chroma if chroma else orc.inter_predict
+        for k in range(len(mvs)):
+            n, o = int(sizes[k]), int(p["out_off"][k])
+            assert np.array_equal(out[o:o + n * n].reshape(n, n), f(ref, int(mvs[k, 0]), int(mvs[k, 1]), n)), \
+                (chroma, k, n, tuple(mvs[k]))
+    engine.free(rp)
+    engine.free(cp)
