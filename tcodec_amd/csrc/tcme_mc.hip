@@ -109,7 +109,7 @@ __device__ __forceinline__ uint32_t as_u32(pk16 v) { return __builtin_bit_cast(u
 
 // ---- luma: aprons of luma bytes staged in LDS ------------------------------
 
-constexpr int kMcChunk = 64;      // descriptors per workgroup
+constexpr int kMcChunk = 128;     // descriptors per workgroup at most
 constexpr int kMcWaveLds = 3328;  // bytes of staged aprons per wavefront (>= every slot's)
 
 typedef unsigned int u32x4a4 __attribute__((ext_vector_type(4), aligned(4)));
@@ -575,7 +575,7 @@ __global__ __launch_bounds__(256) void mc_luma_kernel(const PicDev *__restrict__
         if (s < so) {
             // 128 samples of a block of another size, two per lane
             int k = 0;  // the last block whose first slot is <= s
-            for (int sft = 32; sft; sft >>= 1)
+            for (int sft = kMcChunk / 2; sft; sft >>= 1)
                 if (k + sft < no && oslot[k + sft] <= s) k += sft;
             const McInfo m = info[lists[4][k]];
             const PicDev R = table[m.pic];

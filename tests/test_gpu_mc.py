@@ -405,8 +405,9 @@ def _ctu_lists(rng, w, h, passes, reach, blk_scale=1):
 
 def test_gpu_luma_mc_frame_sized_list(engine, orc):
     """A list as long as a 1080p frame's (142,800 blocks in CTU order over two
-    references): the launch gives each workgroup a chunk of 64 descriptors,
-    so its slots fill (16 8x8 blocks, 4 16x16, 32x32 and 64x64 bands), blocks
+    references): the launch gives each of its 2,048 workgroups a chunk of
+    ~70 descriptors, so its slots fill (16 8x8 blocks, 4 16x16, 32x32 and
+    64x64 bands), blocks
     are sorted by kind with those reaching past the picture first, and runs
     of other sizes and a rejected descriptor sit among them.  Every block
     against the oracle."""
@@ -533,3 +534,30 @@ def test_gpu_mc_extreme_motion_vectors(engine, orc):
                 (chroma, k, n, tuple(mvs[k]))
     engine.free(rp)
     engine.free(cp)
+
+
+def test_gpu_luma_mc_chunks_of_other_sizes(engine, orc):
+    """A list long enough that every workgroup holds a full chunk of 128
+    descriptors (270,000 blocks), nearly all of sizes other than 8/16/32/64
+    (1..13), so a workgroup's slots of 128 samples for those blocks number
+    more than 64 and are found by a binary search over all of them; every
+    block against the oracle."""
+    h, w = 144, 256
+    print(f"luma mc chunks of other sizes {w}x{h}", flush=True)
+    rng = np.random.default_rng(61)
+    ref = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    rp = engine.pic(ref)
+    count = 270_000
+    sizes = rng.choice([1, 2, 3, 4, 5, 7, 12, 13, 8], count, p=[.1, .1, .1, .2, .1, .1, .1, .1, .1])
+    mvs = np.stack([rng.integers(-40, 4 * w + 40, count), rng.integers(-40, 4 * h + 40, count)], 1)
+    preds = _preds(rp, mvs, sizes)
+    out, st = engine.predict(preds)
+    assert (st == 0).all()
+    want = orc.inter_predict_list(ref, mvs, sizes, preds["out_off"], np.zeros_like(out))
+    if not np.array_equal(out, want):
+        n2 = sizes.astype(np.int64) ** 2
+        offs = preds["out_off"]
+        bad = [k for k in range(count) if not np.array_equal(out[offs[k]:offs[k] + n2[k]],
+                                                              want[offs[k]:offs[k] + n2[k]])]
+        assert not bad, [(k, int(sizes[k]), tuple(mvs[k])) for k in bad[:8]]
+    engine.free(rp)
