@@ -107,6 +107,27 @@ def test_inter_predict_matches_reference(orc, ref_lib):
                               ref_lib.inter_predict_chroma(ref, mvx, mvy, n))
 
 
+def test_inter_predict_list_equals_block_by_block(orc):
+    """The list form the GPU tests check against writes each block exactly
+    where, and as, the one-block form does, and nothing else."""
+    rng = np.random.default_rng(12)
+    h, w = 37, 53
+    ref = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    for chroma in (False, True):
+        sizes = rng.choice([1, 3, 4, 8, 16], 300)
+        mvs = np.stack([rng.integers(-90, 4 * w + 90, 300), rng.integers(-90, 4 * h + 90, 300)], 1)
+        offs = np.concatenate([[7], 7 + np.cumsum(sizes.astype(np.int64) ** 2 + 5)[:-1]])
+        out = np.zeros(int(offs[-1] + sizes[-1] ** 2 + 9), np.uint8)
+        orc.inter_predict_list(ref, mvs, sizes, offs, out, chroma=chroma)
+        f = orc.inter_predict_chroma if chroma else orc.inter_predict
+        covered = np.zeros(out.size, bool)
+        for k in range(300):
+            n, o = int(sizes[k]), int(offs[k])
+            assert np.array_equal(out[o:o + n * n].reshape(n, n), f(ref, int(mvs[k, 0]), int(mvs[k, 1]), n))
+            covered[o:o + n * n] = True
+        assert not out[~covered].any()
+
+
 def test_mvp_matches_reference(orc, ref_lib):
     rng = np.random.default_rng(4)
     for trial in range(4000):
